@@ -1,0 +1,174 @@
+/*
+ * mapfold.h -- C ABI of the MI355X map-folding engine (libmapfold.so).
+ *
+ * Plain C types only: pointers, sizes, integers.  Every entry point returns an
+ * int status (MF_OK = 0) and never throws across the boundary; mf_status_string
+ * names a status.  All device work runs on one HIP stream owned by the library
+ * (default priority) on the device chosen by mf_init.  Calls are not
+ * thread-safe: one host thread drives one library context.
+ *
+ * Reference interfaces replaced (johnowhitaker/map_folding, file:line):
+ *   mf_gpu_folding        gpu_folding.cu:209-311  (main: frontier -> launch -> total)
+ *   mf_lunnon_frontier    gpu_folding_v2.cu:220-256, mf_fast.cpp:185-221 (collect_states)
+ *   mf_lunnon_solve       gpu_folding_v2.cu:381-453 (solve_kernel / solve_state_device),
+ *                         gpu_folding.cu:123-207 (compute_counts)
+ *   mf_sym_seeds          sym_fold.cpp:689-710 (recurse_generate + deduplicate_cases)
+ *   mf_sym_cases          sym_fold.cpp:491-527 (recurse_stream_cases, case-index window)
+ *   mf_sym_solve          sym_fold.cpp:219-233,410-489,529-620 (recurse_count over cases)
+ *   mf_sym_fold           sym_fold.cpp:649-774 (main: the decimal answer)
+ *   mf_wasm_*             mfah/wasm/sym_kernel.c:20-34,251-277 (batch case ABI)
+ */
+#ifndef MAPFOLD_H
+#define MAPFOLD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    MF_OK = 0,
+    MF_ERR_ARG = 1,        /* bad dimensions / sizes (cf. sym_kernel.c status 1) */
+    MF_ERR_DEPTH = 2,      /* bad depth / level window (cf. sym_kernel.c status 2) */
+    MF_ERR_HIP = 3,        /* a HIP runtime call failed (message via mf_last_error) */
+    MF_ERR_NO_DEVICE = 4,  /* no GPU visible / mf_init not called */
+    MF_ERR_CAPACITY = 5,   /* output buffer too small or work queue overflow */
+    MF_ERR_INTERNAL = 6    /* device-side consistency check failed */
+};
+
+const char *mf_status_string(int status);
+/* Text of the last error (HIP error string or argument message); never NULL. */
+const char *mf_last_error(void);
+
+/* ---- library context ---------------------------------------------------- */
+/* Select the device and create the library's stream.  Idempotent per device. */
+int mf_init(int device);
+/* Number of compute units of the current device (0 before mf_init). */
+int mf_device_cus(void);
+/* Release device buffers and the stream. */
+void mf_shutdown(void);
+
+/* ---- results ------------------------------------------------------------ */
+/* Exact totals.  `count` is the sum over work items of multiplier x subtree
+ * count as 128 bits (lo, hi).  Event counters are exact and independent of
+ * scheduling:
+ *   sym:    nodes = get_good calls, walks = page walks, steps = walk steps,
+ *           leaves = sum of popcount at the last level
+ *   Lunnon: nodes = leaf inserts, walks = gap computations, steps = walk
+ *           steps, leaves = completions (each worth +n), cands = distinct
+ *           gap candidates before the filter (mf.c:99-123)                   */
+typedef struct {
+    uint64_t count_lo;
+    uint64_t count_hi;
+    uint64_t nodes;
+    uint64_t walks;
+    uint64_t steps;
+    uint64_t leaves;
+    uint64_t cands;        /* Lunnon: gap candidates before the filter; sym: 0 */
+    uint64_t items;        /* work items executed (>= input items when re-split) */
+    uint32_t rounds;       /* kernel rounds (1 for the static schedule) */
+    uint32_t status;
+    double kernel_ms;      /* device time of the solve kernels (HIP events) */
+    double total_ms;       /* host wall time of the call */
+} mf_result;
+
+/* Scheduling options.  Zero-initialise and set what you need; zeros select
+ * tuned defaults.  Scheduling never changes any count. */
+enum { MF_SCHED_STATIC = 0, MF_SCHED_RESPLIT = 1 };
+typedef struct {
+    int32_t schedule;         /* MF_SCHED_STATIC: one lane per item, one launch;
+                                 MF_SCHED_RESPLIT: persistent lanes + on-device re-split */
+    int32_t block;            /* threads per workgroup (64..256) */
+    int32_t waves_per_cu;     /* persistent schedule: resident waves per CU */
+    int32_t budget;           /* re-split: nodes a lane runs on one item before it may split */
+    int32_t split_levels;     /* re-split: at most this many levels below an item's root donate */
+    int32_t count_events;     /* 1: count walks/steps too (nodes/leaves always counted) */
+    uint64_t queue_capacity;  /* re-split: work items the device queue can hold */
+    const uint32_t *order;    /* optional host permutation of item indices (learned order);
+                                 NULL = input order.  Never changes any count. */
+} mf_solve_options;
+
+/* ---- Lunnon DFS (mf.c / gpu_folding*.cu) --------------------------------- */
+/* Canonical frontier at partition depth D (1 <= D <= n): every node at leaf
+ * D+1 passing the normal-form filter, in DFS order.  State s is written as
+ * b[0..D] (D+1 bytes) at out_b + s*(D+1).  *num_states receives the frontier
+ * size even when it exceeds max_states (then only max_states rows are written
+ * and MF_ERR_CAPACITY is returned).  out_b may be NULL to count only. */
+int mf_lunnon_frontier(const int32_t *dims, int32_t ndim, int32_t depth,
+                       uint8_t *out_b, int64_t max_states, int64_t *num_states);
+
+/* Resume `num` frontier states on the GPU and count all completions under
+ * each (solve_state_device semantics: stop at the state's own level).
+ * per_count / per_nodes (nullable) receive each state's count and inserts. */
+int mf_lunnon_solve(const int32_t *dims, int32_t ndim, int32_t depth,
+                    const uint8_t *states_b, int64_t num,
+                    const mf_solve_options *opt,
+                    uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
+
+/* gpu_folding <grid> <depth>: frontier + GPU solve; the total of `mf dims`. */
+int mf_gpu_folding(const int32_t *dims, int32_t ndim, int32_t depth,
+                   const mf_solve_options *opt, mf_result *out, int64_t *num_states);
+
+/* Batched uneven instances: `nmaps` 2-D maps (dims2[2*k], dims2[2*k+1]), each
+ * with its own frontier depth, all resumed in one queue.  per_map_count
+ * receives each map's exact total (the value `mf p q` prints). */
+int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps,
+                    const mf_solve_options *opt, uint64_t *per_map_count, mf_result *out);
+
+/* ---- symmetry-reduced cycle DFS (sym_fold.cpp) --------------------------- */
+/* Seeds: deduplicated center cycles for 6 <= n <= 8 (64-byte cycles, orbit
+ * multipliers), in generation order.  *raw = center cycles before dedup. */
+int mf_sym_seeds(int32_t n, uint8_t *cycles, uint64_t *mults, int64_t max_seeds,
+                 int64_t *num_seeds, int64_t *raw);
+
+/* Expanded cases [start, start+max_cases) at level `expanded` in the global
+ * stream order of sym_fold (seed order, then DFS order with children in
+ * ascending insertion point).  Uses a per-seed case-count index, so the skip
+ * costs O(#seeds + one seed's subtree) instead of O(start).  *num receives the
+ * number written (fewer than max_cases at the end of the stream). */
+int mf_sym_cases(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
+                 uint8_t *cycles, uint64_t *mults, int64_t *num);
+
+/* Total number of expanded cases at level `expanded`. */
+int mf_sym_case_total(int32_t n, int32_t expanded, uint64_t *total);
+
+/* Count `num` cases (64-byte cycles valid on cells [0, level)) on the GPU.
+ * mults may be NULL (all 1).  out->count = sum mult*count (not times n^2). */
+int mf_sym_solve(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults,
+                 int64_t num, const mf_solve_options *opt,
+                 uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
+
+/* sym_fold <n> [threads] [max_cases] [start_case] with MF_STREAM_CASES semantics
+ * (expanded = MF_EXPANDED_N for n >= 7): the decimal answer as 128 bits. */
+int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
+                const mf_solve_options *opt, uint64_t answer[2], mf_result *out);
+
+/* ---- device-resident frontiers (inputs stay in HBM across solves) ------- */
+typedef struct mf_frontier mf_frontier;
+int mf_sym_frontier_create(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults,
+                           int64_t num, mf_frontier **out);
+int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth,
+                              const uint8_t *states_b, int64_t num, mf_frontier **out);
+/* Solve a resident frontier; per_count/per_nodes are host arrays (nullable). */
+int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt,
+                      uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
+void mf_frontier_destroy(mf_frontier *f);
+
+/* ---- sym_kernel.c-compatible batch ABI (mfah wire format) --------------- */
+/* 112-byte case records: cycle bytes [0, n^2) in spiral order, u64 LE
+ * multiplier at byte 100.  Output words: u128 count LE, u64 nodes, u32 status
+ * (mfah/wasm/sym_kernel.c:240-249).  The node definition matches that file:
+ * one per recurse_count call, stop level included (sym_kernel.c:212-214). */
+uint8_t *mf_wasm_input_ptr(void);
+uint32_t *mf_wasm_output_ptr(void);
+uint32_t mf_wasm_case_stride(void);
+uint32_t mf_wasm_max_cases(void);
+uint32_t mf_wasm_solve_cases(uint32_t n, uint32_t start_i, uint32_t stop_i, uint32_t case_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAPFOLD_H */

@@ -1,0 +1,252 @@
+"""map_folding_amd -- MI355X-native map-folding DFS (Lunnon and symmetry-reduced).
+
+Python front-end over libmapfold.so (include/mapfold.h).  Every count is
+computed by the HIP kernels on the GPU; the host side only generates frontiers
+(the reference's CPU half) and moves buffers.  There is no CPU fallback: if
+the library or a GPU is missing, calls raise.
+
+Reference operations mirrored (johnowhitaker/map_folding):
+    gpu_folding(dims, depth)        gpu_folding.cu:209-311   (`gpu_folding <n> <depth>`)
+    lunnon_frontier(dims, depth)    mf_fast.cpp:185-221 / gpu_folding_v2.cu:220-256
+    lunnon_solve(dims, depth, S)    gpu_folding_v2.cu:381-453 (solve_kernel)
+    sym_seeds(n)                    sym_fold.cpp:689-710
+    sym_cases(n, E, start, count)   sym_fold.cpp:491-527 (MF_STREAM_CASES window)
+    sym_solve(n, level, cases)      sym_fold.cpp:219-233,529-620
+    sym_fold(n, ...)                sym_fold.cpp:649-774 (`sym_fold n threads max start`)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._abi import (MF_SCHED_RESPLIT, MF_SCHED_STATIC, MFError, MFResult, MFSolveOptions, check,
+                   declared_symbols, lib)
+
+__all__ = [
+    "MF_SCHED_STATIC", "MF_SCHED_RESPLIT", "MFError", "MFResult", "SolveOptions", "init",
+    "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
+    "sym_cases", "sym_case_total", "sym_solve", "sym_fold", "SymFrontier", "LunnonFrontier",
+    "declared_symbols",
+]
+
+_P = C.POINTER
+
+
+def _p(arr, ctype):
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(_P(ctype))
+
+
+@dataclass
+class SolveOptions:
+    """Scheduling knobs (never change a count).  Zeros select defaults."""
+    schedule: int = MF_SCHED_RESPLIT
+    block: int = 0
+    waves_per_cu: int = 0
+    budget: int = 0
+    split_levels: int = 0
+    count_events: bool = True
+    queue_capacity: int = 0
+    order: np.ndarray | None = None   # permutation of item indices (learned order)
+
+    def to_c(self):
+        o = MFSolveOptions()
+        o.schedule = int(self.schedule)
+        o.block = int(self.block)
+        o.waves_per_cu = int(self.waves_per_cu)
+        o.budget = int(self.budget)
+        o.split_levels = int(self.split_levels)
+        o.count_events = 1 if self.count_events else 0
+        o.queue_capacity = int(self.queue_capacity)
+        keep = None
+        if self.order is not None:
+            keep = np.ascontiguousarray(self.order, dtype=np.uint32)
+            o.order = _p(keep, C.c_uint32)
+        return o, keep
+
+
+def _opts(options):
+    if options is None:
+        options = SolveOptions()
+    return options.to_c()
+
+
+def init(device: int = 0) -> None:
+    check(lib().mf_init(device), "mf_init")
+
+
+def device_cus() -> int:
+    return int(lib().mf_device_cus())
+
+
+def _dims(dims):
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
+    return d, _p(d, C.c_int32)
+
+
+# ----------------------------------------------------------------------------- Lunnon
+
+
+def lunnon_frontier(dims, depth: int) -> np.ndarray:
+    """Canonical frontier at partition depth D: uint8 [S, D+1] rows b[0..D], DFS order."""
+    d, dp = _dims(dims)
+    num = C.c_int64()
+    check(lib().mf_lunnon_frontier(dp, len(d), depth, None, 0, C.byref(num)), "mf_lunnon_frontier")
+    out = np.zeros((num.value, depth + 1), dtype=np.uint8)
+    check(lib().mf_lunnon_frontier(dp, len(d), depth, _p(out, C.c_uint8), num.value, C.byref(num)),
+          "mf_lunnon_frontier")
+    return out
+
+
+def lunnon_solve(dims, depth: int, states: np.ndarray, options: SolveOptions | None = None,
+                 per_item: bool = False):
+    """Resume frontier states on the GPU.  Returns (MFResult, per-state counts, per-state inserts);
+    the per-state arrays are None unless per_item."""
+    d, dp = _dims(dims)
+    states = np.ascontiguousarray(states, dtype=np.uint8)
+    num = states.shape[0]
+    pc = np.zeros(num, dtype=np.uint64) if per_item else None
+    pn = np.zeros(num, dtype=np.uint64) if per_item else None
+    res = MFResult()
+    o, keep = _opts(options)
+    check(lib().mf_lunnon_solve(dp, len(d), depth, _p(states, C.c_uint8), num, C.byref(o),
+                                _p(pc, C.c_uint64), _p(pn, C.c_uint64), C.byref(res)), "mf_lunnon_solve")
+    del keep
+    return res, pc, pn
+
+
+def gpu_folding(dims, depth: int, options: SolveOptions | None = None):
+    """`gpu_folding` for any dims (square in the reference): (total, #states, MFResult)."""
+    d, dp = _dims(dims)
+    res = MFResult()
+    ns = C.c_int64()
+    o, keep = _opts(options)
+    check(lib().mf_gpu_folding(dp, len(d), depth, C.byref(o), C.byref(res), C.byref(ns)), "mf_gpu_folding")
+    return res.count, int(ns.value), res
+
+
+def lunnon_batch(shapes, depths, options: SolveOptions | None = None):
+    """Many 2-D maps in one queue: (per-map totals, MFResult)."""
+    shapes = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 2))
+    depths = np.ascontiguousarray(np.asarray(depths, dtype=np.int32))
+    nm = shapes.shape[0]
+    out = np.zeros(nm, dtype=np.uint64)
+    res = MFResult()
+    o, keep = _opts(options)
+    check(lib().mf_lunnon_batch(_p(shapes, C.c_int32), _p(depths, C.c_int32), nm, C.byref(o),
+                                _p(out, C.c_uint64), C.byref(res)), "mf_lunnon_batch")
+    return [int(x) for x in out], res
+
+
+# ----------------------------------------------------------------------------- sym
+
+
+def sym_seeds(n: int):
+    """(cycles uint8 [S, 64], multipliers uint64 [S], raw center count)."""
+    ns, raw = C.c_int64(), C.c_int64()
+    check(lib().mf_sym_seeds(n, None, None, 0, C.byref(ns), C.byref(raw)), "mf_sym_seeds")
+    cyc = np.zeros((ns.value, 64), dtype=np.uint8)
+    mul = np.zeros(ns.value, dtype=np.uint64)
+    check(lib().mf_sym_seeds(n, _p(cyc, C.c_uint8), _p(mul, C.c_uint64), ns.value, C.byref(ns), C.byref(raw)),
+          "mf_sym_seeds")
+    return cyc, mul, int(raw.value)
+
+
+def sym_cases(n: int, expanded: int, start: int, count: int):
+    """Cases [start, start+count) of the stream at level `expanded`: (cycles [k,64], mults [k])."""
+    cyc = np.zeros((count, 64), dtype=np.uint8)
+    mul = np.zeros(count, dtype=np.uint64)
+    got = C.c_int64()
+    check(lib().mf_sym_cases(n, expanded, start, count, _p(cyc, C.c_uint8), _p(mul, C.c_uint64), C.byref(got)),
+          "mf_sym_cases")
+    return cyc[:got.value], mul[:got.value]
+
+
+def sym_case_total(n: int, expanded: int) -> int:
+    t = C.c_uint64()
+    check(lib().mf_sym_case_total(n, expanded, C.byref(t)), "mf_sym_case_total")
+    return int(t.value)
+
+
+def sym_solve(n: int, level: int, cycles: np.ndarray, mults=None, options: SolveOptions | None = None,
+              per_item: bool = False):
+    """Count cases on the GPU: (MFResult with count = sum mult*count, per-case counts, per-case nodes)."""
+    cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+    num = cycles.shape[0]
+    if mults is not None:
+        mults = np.ascontiguousarray(mults, dtype=np.uint64)
+    pc = np.zeros(num, dtype=np.uint64) if per_item else None
+    pn = np.zeros(num, dtype=np.uint64) if per_item else None
+    res = MFResult()
+    o, keep = _opts(options)
+    check(lib().mf_sym_solve(n, level, _p(cycles, C.c_uint8), _p(mults, C.c_uint64), num, C.byref(o),
+                             _p(pc, C.c_uint64), _p(pn, C.c_uint64), C.byref(res)), "mf_sym_solve")
+    del keep
+    return res, pc, pn
+
+
+def sym_fold(n: int, expanded: int = 24, start: int = 0, max_cases: int = 0,
+             options: SolveOptions | None = None):
+    """The decimal answer of `sym_fold n threads max_cases start_case` (MF_STREAM_CASES)."""
+    ans = np.zeros(2, dtype=np.uint64)
+    res = MFResult()
+    o, keep = _opts(options)
+    check(lib().mf_sym_fold(n, expanded, start, max_cases, C.byref(o), _p(ans, C.c_uint64), C.byref(res)),
+          "mf_sym_fold")
+    return int(ans[0]) + (int(ans[1]) << 64), res
+
+
+# ----------------------------------------------------------------------------- resident frontiers
+
+
+class _Frontier:
+    def __init__(self, handle, num):
+        self._h = handle
+        self.num = num
+
+    def solve(self, options: SolveOptions | None = None, per_item: bool = False):
+        pc = np.zeros(self.num, dtype=np.uint64) if per_item else None
+        pn = np.zeros(self.num, dtype=np.uint64) if per_item else None
+        res = MFResult()
+        o, keep = _opts(options)
+        check(lib().mf_frontier_solve(self._h, C.byref(o), _p(pc, C.c_uint64), _p(pn, C.c_uint64),
+                                      C.byref(res)), "mf_frontier_solve")
+        del keep
+        return res, pc, pn
+
+    def close(self):
+        if self._h:
+            lib().mf_frontier_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SymFrontier(_Frontier):
+    """Cases resident in HBM (solve repeatedly without re-uploading)."""
+
+    def __init__(self, n: int, level: int, cycles: np.ndarray, mults=None):
+        cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+        if mults is not None:
+            mults = np.ascontiguousarray(mults, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().mf_sym_frontier_create(n, level, _p(cycles, C.c_uint8), _p(mults, C.c_uint64),
+                                           cycles.shape[0], C.byref(h)), "mf_sym_frontier_create")
+        super().__init__(h, cycles.shape[0])
+
+
+class LunnonFrontier(_Frontier):
+    def __init__(self, dims, depth: int, states: np.ndarray):
+        d, dp = _dims(dims)
+        states = np.ascontiguousarray(states, dtype=np.uint8)
+        h = C.c_void_p()
+        check(lib().mf_lunnon_frontier_create(dp, len(d), depth, _p(states, C.c_uint8), states.shape[0],
+                                              C.byref(h)), "mf_lunnon_frontier_create")
+        super().__init__(h, states.shape[0])

@@ -1,0 +1,820 @@
+// abi.cpp -- libmapfold.so: the C ABI of include/mapfold.h.
+//
+// Host side of the hot path: builds work items from a frontier, keeps them in
+// HBM, launches the HIP kernels on the library's stream and reduces the exact
+// totals.  Re-split schedule: rounds of the persistent kernel; each round's
+// spill queue is the next round's input until a round spills nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mapfold.h"
+#include "mf_device.hpp"
+#include "mf_host.hpp"
+
+namespace mf {
+void launch_sym_static(const SymArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
+void launch_sym_persistent(const SymArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
+uint32_t sym_lds_bytes(uint32_t levels, uint32_t block);
+const void *sym_kernel_ptr(bool persistent, bool events);
+void launch_lunnon_static(const LunnonArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
+void launch_lunnon_persistent(const LunnonArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
+uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block);
+const void *lunnon_kernel_ptr(bool persistent, bool events);
+}  // namespace mf
+
+using namespace mf;
+
+namespace {
+
+std::string g_err = "";
+
+int fail(int status, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return status;
+}
+
+#define HIP_TRY(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(MF_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return MF_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) return fail(MF_ERR_HIP, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        bytes = want;
+        return MF_OK;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+struct Context {
+    int device = -1;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf items, spill_a, spill_b, acc, counters, per_count, per_nodes, per_map, tables;
+    unsigned long long *host_acc = nullptr;   // pinned
+    uint32_t *host_counters = nullptr;        // pinned: [claim, spill_count]
+};
+
+Context g;
+
+int require_init() {
+    if (g.device < 0) {
+        int rc = mf_init(0);
+        if (rc != MF_OK) return rc;
+    }
+    return MF_OK;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+struct Tuned {
+    int schedule;
+    uint32_t block;
+    uint32_t waves_per_cu;
+    uint32_t budget;
+    uint32_t tail;
+    bool events;
+    uint64_t capacity;
+};
+
+Tuned tune(const mf_solve_options *opt) {
+    Tuned t;
+    t.schedule = opt ? opt->schedule : MF_SCHED_RESPLIT;
+    t.block = (opt && opt->block > 0) ? static_cast<uint32_t>(opt->block) : 256u;
+    t.block = std::max(64u, std::min(256u, (t.block / 64u) * 64u));
+    t.waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
+    t.budget = (opt && opt->budget > 0) ? static_cast<uint32_t>(opt->budget) : 64u;
+    t.tail = (opt && opt->split_levels > 0) ? static_cast<uint32_t>(opt->split_levels) * t.budget : t.budget;
+    t.events = opt ? opt->count_events != 0 : true;
+    t.capacity = (opt && opt->queue_capacity > 0) ? opt->queue_capacity : 0;
+    return t;
+}
+
+// Persistent grid: as many blocks as fit (LDS- and wave-limited) on every CU.
+int persistent_grid(const void *kernel, uint32_t block, uint32_t lds, uint32_t waves_per_cu, uint32_t *grid) {
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, static_cast<int>(block), lds));
+    if (per_cu <= 0) return fail(MF_ERR_ARG, "kernel does not fit: block %u with %u bytes of LDS", block, lds);
+    const uint32_t by_waves = std::max(1u, waves_per_cu / (block / 64u));
+    const uint32_t per = std::min(static_cast<uint32_t>(per_cu), by_waves);
+    *grid = per * static_cast<uint32_t>(g.cus);
+    return MF_OK;
+}
+
+int set_lds_limit(const void *kernel, uint32_t lds) {
+    HIP_TRY(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    return MF_OK;
+}
+
+void fill_result(mf_result *out, const unsigned long long *acc) {
+    out->count_lo = acc[ACC_COUNT_LO];
+    out->count_hi = acc[ACC_COUNT_HI];
+    out->nodes = acc[ACC_NODES];
+    out->walks = acc[ACC_WALKS];
+    out->steps = acc[ACC_STEPS];
+    out->leaves = acc[ACC_LEAVES];
+    out->cands = acc[ACC_CANDS];
+    out->items = acc[ACC_ITEMS];
+}
+
+// Generic solve driver over a device-resident item array.
+//   Launch(args_items, num, spill_target, grid, persistent) runs one launch.
+template <typename Args, typename LaunchS, typename LaunchP>
+int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds_bytes, const Tuned &t,
+              const WorkItem *dev_items, uint64_t num, uint64_t origins, uint64_t *per_count, uint64_t *per_nodes,
+              uint64_t *per_map, uint32_t nmaps, mf_result *out, LaunchS launch_s, LaunchP launch_p) {
+    const auto t0 = std::chrono::steady_clock::now();
+    std::memset(out, 0, sizeof(*out));
+    int rc;
+    if ((rc = g.acc.ensure(ACC_SLOTS * sizeof(unsigned long long))) != MF_OK) return rc;
+    if ((rc = g.counters.ensure(4 * sizeof(uint32_t))) != MF_OK) return rc;
+    HIP_TRY(hipMemsetAsync(g.acc.ptr, 0, ACC_SLOTS * sizeof(unsigned long long), g.stream));
+    unsigned long long *dcount = nullptr, *dnodes = nullptr, *dmap = nullptr;
+    if (per_count) {
+        if ((rc = g.per_count.ensure(origins * 8)) != MF_OK) return rc;
+        HIP_TRY(hipMemsetAsync(g.per_count.ptr, 0, origins * 8, g.stream));
+        dcount = static_cast<unsigned long long *>(g.per_count.ptr);
+    }
+    if (per_nodes) {
+        if ((rc = g.per_nodes.ensure(origins * 8)) != MF_OK) return rc;
+        HIP_TRY(hipMemsetAsync(g.per_nodes.ptr, 0, origins * 8, g.stream));
+        dnodes = static_cast<unsigned long long *>(g.per_nodes.ptr);
+    }
+    if (per_map) {
+        if ((rc = g.per_map.ensure(static_cast<size_t>(nmaps) * 8)) != MF_OK) return rc;
+        HIP_TRY(hipMemsetAsync(g.per_map.ptr, 0, static_cast<size_t>(nmaps) * 8, g.stream));
+        dmap = static_cast<unsigned long long *>(g.per_map.ptr);
+    }
+    args.acc = static_cast<unsigned long long *>(g.acc.ptr);
+    args.per_count = dcount;
+    args.per_nodes = dnodes;
+    args.budget = t.budget;
+    args.tail = t.tail;
+    uint32_t *claim = static_cast<uint32_t *>(g.counters.ptr);
+    uint32_t *spill_count = claim + 1;
+    args.claim = claim;
+    args.spill_count = spill_count;
+
+    uint32_t rounds = 0;
+    if (num > 0) {
+        if (t.schedule == MF_SCHED_STATIC) {
+            if ((rc = set_lds_limit(kstatic, lds_bytes)) != MF_OK) return rc;
+            args.items = dev_items;
+            args.num_items = static_cast<uint32_t>(num);
+            args.spill = nullptr;
+            args.spill_cap = 0;
+            const uint32_t grid = static_cast<uint32_t>((num + t.block - 1) / t.block);
+            HIP_TRY(hipEventRecord(g.ev0, g.stream));
+            launch_s(args, grid, t.block, lds_bytes, t.events, g.stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(g.ev1, g.stream));
+            rounds = 1;
+        } else {
+            if ((rc = set_lds_limit(kpersist, lds_bytes)) != MF_OK) return rc;
+            uint32_t grid = 0;
+            if ((rc = persistent_grid(kpersist, t.block, lds_bytes, t.waves_per_cu, &grid)) != MF_OK) return rc;
+            const uint64_t lanes = static_cast<uint64_t>(grid) * t.block;
+            // A lane spills at most once per round: its current node plus one
+            // sibling set per stack level.
+            uint64_t cap = t.capacity ? t.capacity : lanes * 65ull;
+            cap = std::max<uint64_t>(cap, lanes * 2);
+            if (cap > 0xffffffffull) cap = 0xffffffffull;
+            if ((rc = g.spill_a.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
+            if ((rc = g.spill_b.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
+            WorkItem *bufs[2] = {static_cast<WorkItem *>(g.spill_a.ptr), static_cast<WorkItem *>(g.spill_b.ptr)};
+            const WorkItem *in = dev_items;
+            uint64_t in_num = num;
+            int next = 0;
+            HIP_TRY(hipEventRecord(g.ev0, g.stream));
+            while (in_num > 0) {
+                if (rounds >= 100000) return fail(MF_ERR_INTERNAL, "re-split did not converge");
+                HIP_TRY(hipMemsetAsync(claim, 0, 2 * sizeof(uint32_t), g.stream));
+                args.items = in;
+                args.num_items = static_cast<uint32_t>(in_num);
+                args.spill = bufs[next];
+                args.spill_cap = static_cast<uint32_t>(cap);
+                launch_p(args, grid, t.block, lds_bytes, t.events, g.stream);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipMemcpyAsync(g.host_counters, claim, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+                HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, args.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost, g.stream));
+                HIP_TRY(hipStreamSynchronize(g.stream));
+                ++rounds;
+                if (g.host_acc[ACC_ERROR] & 2ull) return fail(MF_ERR_INTERNAL, "a state walk did not close");
+                if (g.host_acc[ACC_ERROR] != 0)
+                    return fail(MF_ERR_CAPACITY, "spill queue overflow (capacity %llu items)",
+                                static_cast<unsigned long long>(cap));
+                in = bufs[next];
+                in_num = g.host_counters[1];
+                next ^= 1;
+            }
+            HIP_TRY(hipEventRecord(g.ev1, g.stream));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(g.host_acc, g.acc.ptr, ACC_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           g.stream));
+    if (per_count) HIP_TRY(hipMemcpyAsync(per_count, dcount, origins * 8, hipMemcpyDeviceToHost, g.stream));
+    if (per_nodes) HIP_TRY(hipMemcpyAsync(per_nodes, dnodes, origins * 8, hipMemcpyDeviceToHost, g.stream));
+    if (per_map) HIP_TRY(hipMemcpyAsync(per_map, dmap, static_cast<size_t>(nmaps) * 8, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (g.host_acc[ACC_ERROR] & 2ull) return fail(MF_ERR_INTERNAL, "a state walk did not close");
+    if (g.host_acc[ACC_ERROR] != 0) return fail(MF_ERR_CAPACITY, "work queue overflow");
+    fill_result(out, g.host_acc);
+    out->rounds = rounds;
+    if (num > 0) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+        out->kernel_ms = ms;
+    }
+    out->total_ms = ms_since(t0);
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------ frontiers
+
+}  // namespace
+
+struct mf_frontier {
+    int kind = 0;                  // 1 = sym, 2 = Lunnon
+    SymProblem sym;
+    int level = 0;
+    int stop = 0;                  // sym: paths are counted at this level (n2 = full foldings)
+    std::vector<LunnonProblem> maps;
+    std::vector<int> depths;
+    DevBuf items;
+    uint64_t num = 0;
+    std::vector<WorkItem> host_items;   // kept for learned reordering
+};
+
+namespace {
+
+int sym_args(const mf_frontier &f, SymArgs &a, uint32_t *levels) {
+    std::memset(&a, 0, sizeof(a));
+    std::memcpy(a.edge_word, f.sym.edge_word, sizeof(a.edge_word));
+    a.last = static_cast<uint32_t>((f.stop > 0 ? f.stop : f.sym.n2) - 1);
+    a.base_level = static_cast<uint32_t>(f.level);
+    *levels = a.last - a.base_level;
+    if (*levels == 0) *levels = 1;
+    a.levels = *levels;
+    return MF_OK;
+}
+
+// Build the Lunnon table blob and map descriptors; returns stack levels.
+int lunnon_args(const mf_frontier &f, LunnonArgs &a, std::vector<uint8_t> &blob, uint32_t *levels) {
+    std::memset(&a, 0, sizeof(a));
+    if (f.maps.size() > static_cast<size_t>(kMaxMaps)) return fail(MF_ERR_ARG, "at most %d maps per launch", kMaxMaps);
+    blob.clear();
+    uint32_t lv = 1;
+    for (size_t k = 0; k < f.maps.size(); ++k) {
+        const LunnonProblem &pr = f.maps[k];
+        LunnonMap m;
+        m.table_off = static_cast<uint32_t>(blob.size());
+        m.n = static_cast<uint16_t>(pr.n);
+        m.ndim = static_cast<uint8_t>(pr.ndim);
+        m.base = static_cast<uint8_t>(f.depths[k] + 1);
+        a.maps[k] = m;
+        blob.insert(blob.end(), pr.d.begin(), pr.d.end());
+        lv = std::max<uint32_t>(lv, static_cast<uint32_t>(pr.n - f.depths[k] - 1));
+    }
+    a.num_maps = static_cast<uint32_t>(f.maps.size());
+    a.tables_bytes = static_cast<uint32_t>(blob.size());
+    a.levels = lv;
+    *levels = lv;
+    return MF_OK;
+}
+
+int upload_items(mf_frontier *f) {
+    int rc;
+    if ((rc = f->items.ensure(std::max<size_t>(1, f->host_items.size()) * sizeof(WorkItem))) != MF_OK) return rc;
+    if (!f->host_items.empty())
+        HIP_TRY(hipMemcpyAsync(f->items.ptr, f->host_items.data(), f->host_items.size() * sizeof(WorkItem),
+                               hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    f->num = f->host_items.size();
+    return MF_OK;
+}
+
+int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_count, uint64_t *per_nodes,
+                   uint64_t *per_map, mf_result *out) {
+    int rc = require_init();
+    if (rc != MF_OK) return rc;
+    const Tuned t = tune(opt);
+    // Learned / caller order: permute the resident items once on the host
+    // side; origins stay attached to each item so results are unchanged.
+    const WorkItem *dev_items = static_cast<const WorkItem *>(f->items.ptr);
+    DevBuf reordered;
+    if (opt && opt->order && f->num > 0) {
+        std::vector<WorkItem> tmp(f->num);
+        for (uint64_t k = 0; k < f->num; ++k) {
+            const uint32_t src = opt->order[k];
+            if (src >= f->num) return fail(MF_ERR_ARG, "order[%llu] out of range", static_cast<unsigned long long>(k));
+            tmp[k] = f->host_items[src];
+        }
+        if ((rc = reordered.ensure(f->num * sizeof(WorkItem))) != MF_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(reordered.ptr, tmp.data(), f->num * sizeof(WorkItem), hipMemcpyHostToDevice, g.stream));
+        dev_items = static_cast<const WorkItem *>(reordered.ptr);
+    }
+    if (f->kind == 1) {
+        SymArgs a;
+        uint32_t levels = 0;
+        sym_args(*f, a, &levels);
+        const uint32_t lds = sym_lds_bytes(levels, t.block);
+        rc = run_solve(a, sym_kernel_ptr(false, t.events), sym_kernel_ptr(true, t.events), lds, t, dev_items, f->num,
+                       f->num, per_count, per_nodes, nullptr, 0, out, launch_sym_static, launch_sym_persistent);
+    } else {
+        LunnonArgs a;
+        std::vector<uint8_t> blob;
+        uint32_t levels = 0;
+        if ((rc = lunnon_args(*f, a, blob, &levels)) != MF_OK) return rc;
+        if ((rc = g.tables.ensure(std::max<size_t>(blob.size(), 16))) != MF_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(g.tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream));
+        a.tables = static_cast<const uint8_t *>(g.tables.ptr);
+        a.per_map = nullptr;
+        const uint32_t lds = lunnon_lds_bytes(a.tables_bytes, levels, t.block);
+        if (per_map) {
+            if ((rc = g.per_map.ensure(f->maps.size() * 8)) != MF_OK) return rc;
+            a.per_map = static_cast<unsigned long long *>(g.per_map.ptr);
+        }
+        rc = run_solve(a, lunnon_kernel_ptr(false, t.events), lunnon_kernel_ptr(true, t.events), lds, t, dev_items,
+                       f->num, f->num, per_count, per_nodes, per_map, static_cast<uint32_t>(f->maps.size()), out,
+                       launch_lunnon_static, launch_lunnon_persistent);
+    }
+    reordered.release();
+    return rc;
+}
+
+// A case must be one cycle through exactly the cells [0, level): then every
+// get_good walk is an orbit of a permutation and terminates.
+bool valid_cycle(const uint8_t *cycle, int level) {
+    uint64_t seen = 0;
+    int c = 0;
+    for (int k = 0; k < level; ++k) {
+        if (cycle[c] >= level) return false;
+        seen |= 1ull << c;
+        c = cycle[c];
+    }
+    return c == 0 && seen == ((level >= 64) ? ~0ull : ((1ull << level) - 1ull));
+}
+
+// A Lunnon state must be one circular list through leaves [0, depth] with
+// leaf 1 directly below the root (the normal form of the frontier).
+bool valid_list(const uint8_t *b, int depth) {
+    return depth >= 1 && b[0] == 1 && valid_cycle(b, depth + 1);
+}
+
+WorkItem sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
+    WorkItem it;
+    std::memset(&it, 0, sizeof(it));
+    std::memcpy(it.state, cycle, 64);
+    it.mask = 0;
+    it.origin = origin;
+    it.mult = static_cast<uint16_t>(mult);
+    it.level = static_cast<uint8_t>(level);
+    it.map = 0;
+    return it;
+}
+
+WorkItem lunnon_item(const uint8_t *b, int depth, int n, uint32_t origin, int map) {
+    WorkItem it;
+    std::memset(&it, 0, sizeof(it));
+    std::memcpy(it.state, b, static_cast<size_t>(depth + 1));
+    it.mask = 0;
+    it.origin = origin;
+    it.mult = static_cast<uint16_t>(n);
+    it.level = static_cast<uint8_t>(depth + 1);
+    it.map = static_cast<uint8_t>(map);
+    return it;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *mf_status_string(int status) {
+    switch (status) {
+        case MF_OK: return "ok";
+        case MF_ERR_ARG: return "invalid argument";
+        case MF_ERR_DEPTH: return "invalid depth";
+        case MF_ERR_HIP: return "HIP runtime error";
+        case MF_ERR_NO_DEVICE: return "no GPU device";
+        case MF_ERR_CAPACITY: return "capacity exceeded";
+        case MF_ERR_INTERNAL: return "internal consistency failure";
+        default: return "unknown status";
+    }
+}
+
+const char *mf_last_error(void) { return g_err.c_str(); }
+
+int mf_init(int device) {
+    if (g.device == device) return MF_OK;
+    if (g.device >= 0) mf_shutdown();
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(MF_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count) return fail(MF_ERR_NO_DEVICE, "device %d out of range (%d visible)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceGetAttribute(&g.cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&g.ev0));
+    HIP_TRY(hipEventCreate(&g.ev1));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_acc), ACC_SLOTS * sizeof(unsigned long long)));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_counters), 4 * sizeof(uint32_t)));
+    g.device = device;
+    return MF_OK;
+}
+
+int mf_device_cus(void) { return g.device >= 0 ? g.cus : 0; }
+
+void mf_shutdown(void) {
+    if (g.device < 0) return;
+    (void)hipSetDevice(g.device);
+    if (g.stream) (void)hipStreamSynchronize(g.stream);
+    for (DevBuf *b : {&g.items, &g.spill_a, &g.spill_b, &g.acc, &g.counters, &g.per_count, &g.per_nodes, &g.per_map,
+                      &g.tables})
+        b->release();
+    if (g.host_acc) (void)hipHostFree(g.host_acc);
+    if (g.host_counters) (void)hipHostFree(g.host_counters);
+    if (g.ev0) (void)hipEventDestroy(g.ev0);
+    if (g.ev1) (void)hipEventDestroy(g.ev1);
+    if (g.stream) (void)hipStreamDestroy(g.stream);
+    g = Context();
+}
+
+// ----------------------------------------------------------------- Lunnon
+
+int mf_lunnon_frontier(const int32_t *dims, int32_t ndim, int32_t depth, uint8_t *out_b, int64_t max_states,
+                       int64_t *num_states) {
+    LunnonProblem pr;
+    std::string err;
+    if (!dims || !lunnon_problem(dims, ndim, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (depth < 1 || depth > pr.n) return fail(MF_ERR_DEPTH, "depth must be between 1 and %d", pr.n);
+    const int64_t total = lunnon_frontier(pr, depth, out_b, out_b ? max_states : 0);
+    if (num_states) *num_states = total;
+    if (out_b && total > max_states) return fail(MF_ERR_CAPACITY, "frontier has %lld states", static_cast<long long>(total));
+    return MF_OK;
+}
+
+int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth, const uint8_t *states_b, int64_t num,
+                              mf_frontier **out) {
+    int rc = require_init();
+    if (rc != MF_OK) return rc;
+    if (!out || num < 0 || (num > 0 && !states_b)) return fail(MF_ERR_ARG, "bad frontier arguments");
+    LunnonProblem pr;
+    std::string err;
+    if (!dims || !lunnon_problem(dims, ndim, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (depth < 1 || depth >= pr.n) return fail(MF_ERR_DEPTH, "depth must be between 1 and %d", pr.n - 1);
+    if (num > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many states for one frontier");
+    for (int64_t s = 0; s < num; ++s)
+        if (!valid_list(states_b + s * (depth + 1), depth))
+            return fail(MF_ERR_ARG, "state %lld is not a normal-form list over leaves 0..%d", static_cast<long long>(s),
+                        depth);
+    auto *f = new mf_frontier();
+    f->kind = 2;
+    f->maps.push_back(pr);
+    f->depths.push_back(depth);
+    f->host_items.reserve(static_cast<size_t>(num));
+    for (int64_t s = 0; s < num; ++s)
+        f->host_items.push_back(lunnon_item(states_b + s * (depth + 1), depth, pr.n, static_cast<uint32_t>(s), 0));
+    if ((rc = upload_items(f)) != MF_OK) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return MF_OK;
+}
+
+int mf_lunnon_solve(const int32_t *dims, int32_t ndim, int32_t depth, const uint8_t *states_b, int64_t num,
+                    const mf_solve_options *opt, uint64_t *per_count, uint64_t *per_nodes, mf_result *out) {
+    if (!out) return fail(MF_ERR_ARG, "out is NULL");
+    mf_frontier *f = nullptr;
+    int rc = mf_lunnon_frontier_create(dims, ndim, depth, states_b, num, &f);
+    if (rc != MF_OK) return rc;
+    rc = mf_frontier_solve(f, opt, per_count, per_nodes, out);
+    mf_frontier_destroy(f);
+    return rc;
+}
+
+int mf_gpu_folding(const int32_t *dims, int32_t ndim, int32_t depth, const mf_solve_options *opt, mf_result *out,
+                   int64_t *num_states) {
+    if (!out) return fail(MF_ERR_ARG, "out is NULL");
+    LunnonProblem pr;
+    std::string err;
+    if (!dims || !lunnon_problem(dims, ndim, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (depth < 1 || depth >= pr.n) return fail(MF_ERR_DEPTH, "partition depth must be between 1 and %d", pr.n - 1);
+    const int64_t total = lunnon_frontier(pr, depth, nullptr, 0);
+    if (num_states) *num_states = total;
+    std::vector<uint8_t> states(static_cast<size_t>(total) * (depth + 1));
+    lunnon_frontier(pr, depth, states.data(), total);
+    return mf_lunnon_solve(dims, ndim, depth, states.data(), total, opt, nullptr, nullptr, out);
+}
+
+int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps, const mf_solve_options *opt,
+                    uint64_t *per_map_count, mf_result *out) {
+    int rc = require_init();
+    if (rc != MF_OK) return rc;
+    if (!dims2 || !depths || !out || nmaps < 1 || nmaps > kMaxMaps) return fail(MF_ERR_ARG, "bad batch arguments");
+    mf_frontier f;
+    f.kind = 2;
+    for (int k = 0; k < nmaps; ++k) {
+        LunnonProblem pr;
+        std::string err;
+        if (!lunnon_problem(dims2 + 2 * k, 2, pr, err)) return fail(MF_ERR_ARG, "map %d: %s", k, err.c_str());
+        const int d = depths[k];
+        if (d < 1 || d >= pr.n) return fail(MF_ERR_DEPTH, "map %d: depth must be between 1 and %d", k, pr.n - 1);
+        const int64_t cnt = lunnon_frontier(pr, d, nullptr, 0);
+        std::vector<uint8_t> st(static_cast<size_t>(cnt) * (d + 1));
+        lunnon_frontier(pr, d, st.data(), cnt);
+        for (int64_t s = 0; s < cnt; ++s)
+            f.host_items.push_back(lunnon_item(st.data() + s * (d + 1), d, pr.n,
+                                               static_cast<uint32_t>(f.host_items.size()), k));
+        f.maps.push_back(pr);
+        f.depths.push_back(d);
+    }
+    if ((rc = upload_items(&f)) != MF_OK) return rc;
+    rc = solve_frontier(&f, opt, nullptr, nullptr, per_map_count, out);
+    f.items.release();
+    return rc;
+}
+
+// ------------------------------------------------------------------- sym
+
+int mf_sym_seeds(int32_t n, uint8_t *cycles, uint64_t *mults, int64_t max_seeds, int64_t *num_seeds, int64_t *raw) {
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 6) return fail(MF_ERR_ARG, "seeds exist for 6 <= n <= 8");
+    int64_t r = 0;
+    const auto seeds = sym_seeds(pr, &r);
+    if (raw) *raw = r;
+    if (num_seeds) *num_seeds = static_cast<int64_t>(seeds.size());
+    const int64_t k = std::min<int64_t>(max_seeds, static_cast<int64_t>(seeds.size()));
+    for (int64_t s = 0; s < k; ++s) {
+        if (cycles) std::memcpy(cycles + s * 64, seeds[s].cycle, 64);
+        if (mults) mults[s] = seeds[s].mult;
+    }
+    if ((cycles || mults) && max_seeds < static_cast<int64_t>(seeds.size())) return fail(MF_ERR_CAPACITY, "too many seeds");
+    return MF_OK;
+}
+
+static int check_expanded(int n, int expanded) {
+    const int c2 = (n % 2 == 0) ? 16 : 9;
+    if (expanded < c2 || expanded > n * n - 1)
+        return fail(MF_ERR_DEPTH, "MF_EXPANDED_N must be between %d and %d", c2, n * n - 1);
+    return MF_OK;
+}
+
+int mf_sym_cases(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases, uint8_t *cycles, uint64_t *mults,
+                 int64_t *num) {
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 8");
+    int rc = check_expanded(n, expanded);
+    if (rc != MF_OK) return rc;
+    if (max_cases > 0 && (!cycles || !mults)) return fail(MF_ERR_ARG, "output buffers are NULL");
+    const int64_t got = sym_cases(pr, expanded, start, max_cases, cycles, mults);
+    if (num) *num = got;
+    return MF_OK;
+}
+
+int mf_sym_case_total(int32_t n, int32_t expanded, uint64_t *total) {
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 8");
+    int rc = check_expanded(n, expanded);
+    if (rc != MF_OK) return rc;
+    const auto &counts = sym_seed_case_counts(pr, expanded);
+    uint64_t t = 0;
+    for (uint64_t c : counts) t += c;
+    if (total) *total = t;
+    return MF_OK;
+}
+
+static int sym_frontier_create_stop(int32_t n, int32_t level, int32_t stop, const uint8_t *cycles,
+                                    const uint64_t *mults, int64_t num, mf_frontier **out);
+
+int mf_sym_frontier_create(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults, int64_t num,
+                           mf_frontier **out) {
+    return sym_frontier_create_stop(n, level, n * n, cycles, mults, num, out);
+}
+
+static int sym_frontier_create_stop(int32_t n, int32_t level, int32_t stop, const uint8_t *cycles,
+                                    const uint64_t *mults, int64_t num, mf_frontier **out) {
+    int rc = require_init();
+    if (rc != MF_OK) return rc;
+    if (!out || num < 0 || (num > 0 && !cycles)) return fail(MF_ERR_ARG, "bad frontier arguments");
+    if (num > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many cases for one frontier");
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 2) return fail(MF_ERR_ARG, "n must be at least 2 for the cycle DFS");
+    if (stop < 2 || stop > pr.n2) return fail(MF_ERR_DEPTH, "stop level must be between 2 and %d", pr.n2);
+    if (level < 1 || level > stop - 1) return fail(MF_ERR_DEPTH, "level must be between 1 and %d", stop - 1);
+    for (int64_t s = 0; s < num; ++s) {
+        const uint64_t m = mults ? mults[s] : 1;
+        if (m == 0 || m > 0xffff) return fail(MF_ERR_ARG, "multiplier out of range at case %lld", static_cast<long long>(s));
+        if (!valid_cycle(cycles + s * 64, level))
+            return fail(MF_ERR_ARG, "case %lld is not one cycle through cells 0..%d", static_cast<long long>(s), level - 1);
+    }
+    auto *f = new mf_frontier();
+    f->kind = 1;
+    f->sym = pr;
+    f->level = level;
+    f->stop = stop;
+    f->host_items.reserve(static_cast<size_t>(num));
+    for (int64_t s = 0; s < num; ++s)
+        f->host_items.push_back(sym_item(cycles + s * 64, mults ? mults[s] : 1, level, static_cast<uint32_t>(s)));
+    if ((rc = upload_items(f)) != MF_OK) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return MF_OK;
+}
+
+int mf_sym_solve(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults, int64_t num,
+                 const mf_solve_options *opt, uint64_t *per_count, uint64_t *per_nodes, mf_result *out) {
+    if (!out) return fail(MF_ERR_ARG, "out is NULL");
+    mf_frontier *f = nullptr;
+    int rc = mf_sym_frontier_create(n, level, cycles, mults, num, &f);
+    if (rc != MF_OK) return rc;
+    rc = mf_frontier_solve(f, opt, per_count, per_nodes, out);
+    mf_frontier_destroy(f);
+    return rc;
+}
+
+int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases, const mf_solve_options *opt,
+                uint64_t answer[2], mf_result *out) {
+    if (!answer || !out) return fail(MF_ERR_ARG, "answer/out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    if (n < 1 || n > 8) return fail(MF_ERR_ARG, "n must be between 1 and 8");
+    unsigned __int128 ans = 0;
+    if (n == 1) {
+        ans = 1;
+    } else if (n == 2) {
+        ans = 8;
+    } else if (n <= 5) {
+        // One case: the initial cycle 0 -> 1 -> 2 -> 0 at cell 3; both
+        // orientations and n^2 starting cells (sym_fold.cpp:691-702).
+        uint8_t cycle[64] = {1, 2, 0};
+        int rc = mf_sym_solve(n, 3, cycle, nullptr, 1, opt, nullptr, nullptr, out);
+        if (rc != MF_OK) return rc;
+        ans = static_cast<unsigned __int128>(2) * static_cast<unsigned>(n * n) * out->count_lo;
+    } else {
+        const int level = (n >= 7) ? expanded : 16;
+        if (n >= 7) {
+            int rc = check_expanded(n, expanded);
+            if (rc != MF_OK) return rc;
+        }
+        SymProblem pr;
+        std::string err;
+        sym_problem(n, pr, err);
+        // Stream the window in batches that bound host and device memory.
+        const uint64_t batch = 1ull << 22;
+        uint64_t pos = start, left = max_cases == 0 ? ~0ull : max_cases;
+        std::vector<uint8_t> cyc;
+        std::vector<uint64_t> mul;
+        unsigned __int128 sum = 0;
+        mf_result part;
+        while (left > 0) {
+            const uint64_t want = std::min(batch, left);
+            cyc.resize(want * 64);
+            mul.resize(want);
+            const int64_t got = sym_cases(pr, level, pos, want, cyc.data(), mul.data());
+            if (got <= 0) break;
+            int rc = mf_sym_solve(n, level, cyc.data(), mul.data(), got, opt, nullptr, nullptr, &part);
+            if (rc != MF_OK) return rc;
+            sum += (static_cast<unsigned __int128>(part.count_hi) << 64) | part.count_lo;
+            out->nodes += part.nodes;
+            out->walks += part.walks;
+            out->steps += part.steps;
+            out->leaves += part.leaves;
+            out->items += part.items;
+            out->rounds += part.rounds;
+            out->kernel_ms += part.kernel_ms;
+            pos += static_cast<uint64_t>(got);
+            left -= static_cast<uint64_t>(got);
+            if (static_cast<uint64_t>(got) < want) break;
+        }
+        ans = sum * static_cast<unsigned>(n * n);
+    }
+    answer[0] = static_cast<uint64_t>(ans);
+    answer[1] = static_cast<uint64_t>(ans >> 64);
+    return MF_OK;
+}
+
+int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_count, uint64_t *per_nodes,
+                      mf_result *out) {
+    if (!f || !out) return fail(MF_ERR_ARG, "frontier/out is NULL");
+    return solve_frontier(f, opt, per_count, per_nodes, nullptr, out);
+}
+
+void mf_frontier_destroy(mf_frontier *f) {
+    if (!f) return;
+    f->items.release();
+    delete f;
+}
+
+// ------------------------------------------- sym_kernel.c-compatible batch ABI
+// Record layout and status words of mfah/wasm/sym_kernel.c:1-34,240-277:
+// 112-byte records (cycle bytes, u64 LE multiplier at byte 100), output =
+// u128 count LE, u64 nodes, u32 status.  Node definition of that file: one
+// per recurse_count call, the calls that reach stop_i included.
+
+static uint8_t g_wasm_cases[256 * 112];
+static uint32_t g_wasm_out[8];
+
+uint8_t *mf_wasm_input_ptr(void) { return g_wasm_cases; }
+uint32_t *mf_wasm_output_ptr(void) { return g_wasm_out; }
+uint32_t mf_wasm_case_stride(void) { return 112; }
+uint32_t mf_wasm_max_cases(void) { return 256; }
+
+static uint32_t wasm_write(unsigned __int128 count, uint64_t nodes, uint32_t status) {
+    const uint64_t lo = static_cast<uint64_t>(count), hi = static_cast<uint64_t>(count >> 64);
+    g_wasm_out[0] = static_cast<uint32_t>(lo);
+    g_wasm_out[1] = static_cast<uint32_t>(lo >> 32);
+    g_wasm_out[2] = static_cast<uint32_t>(hi);
+    g_wasm_out[3] = static_cast<uint32_t>(hi >> 32);
+    g_wasm_out[4] = static_cast<uint32_t>(nodes);
+    g_wasm_out[5] = static_cast<uint32_t>(nodes >> 32);
+    g_wasm_out[6] = status;
+    g_wasm_out[7] = 0;
+    return status;
+}
+
+uint32_t mf_wasm_solve_cases(uint32_t n, uint32_t start_i, uint32_t stop_i, uint32_t case_count) {
+    // 64-bit masks: squares up to 8x8 (the reference's wasm build also takes 9x9).
+    if (n < 2 || n > 8 || case_count > 256) {
+        fail(MF_ERR_ARG, "n must be 2..8 and case_count <= 256");
+        return wasm_write(0, 0, 1);
+    }
+    const uint32_t n2 = n * n;
+    if (start_i > stop_i || stop_i > n2) {
+        fail(MF_ERR_DEPTH, "need start_i <= stop_i <= n*n");
+        return wasm_write(0, 0, 2);
+    }
+    std::vector<uint8_t> cycles(static_cast<size_t>(case_count) * 64, 0);
+    std::vector<uint64_t> mults(case_count);
+    unsigned __int128 total = 0;
+    for (uint32_t k = 0; k < case_count; ++k) {
+        const uint8_t *rec = g_wasm_cases + k * 112;
+        std::memcpy(cycles.data() + k * 64, rec, n2);
+        uint64_t m = 0;
+        for (int b = 7; b >= 0; --b) m = (m << 8) | rec[100 + b];
+        mults[k] = m;
+    }
+    if (start_i == stop_i || case_count == 0) {
+        for (uint32_t k = 0; k < case_count; ++k) total += mults[k];
+        return wasm_write(total, case_count, 0);
+    }
+    // Multipliers can exceed the kernel's 16-bit field: count with 1 and
+    // weight on the host from the per-case results.
+    mf_frontier *f = nullptr;
+    if (sym_frontier_create_stop(static_cast<int32_t>(n), static_cast<int32_t>(start_i), static_cast<int32_t>(stop_i),
+                                 cycles.data(), nullptr, case_count, &f) != MF_OK)
+        return wasm_write(0, 0, 3);
+    std::vector<uint64_t> per(case_count, 0);
+    mf_result r;
+    mf_solve_options opt;
+    std::memset(&opt, 0, sizeof(opt));
+    opt.schedule = MF_SCHED_RESPLIT;
+    opt.count_events = 0;
+    const int rc = solve_frontier(f, &opt, per.data(), nullptr, nullptr, &r);
+    mf_frontier_destroy(f);
+    if (rc != MF_OK) return wasm_write(0, 0, 3);
+    for (uint32_t k = 0; k < case_count; ++k) total += static_cast<unsigned __int128>(per[k]) * mults[k];
+    return wasm_write(total, r.nodes + r.leaves, 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,368 @@
+// lunnon_kernels.hip -- Lunnon map-folding DFS on gfx950.
+//
+// Restates the resume loop of mf.c:78-143 / solve_state_device
+// (gpu_folding_v2.cu:381-433) per lane:
+//   * state = the "below" links b[0..l-1] of the placed leaves (a circular
+//     list rooted at leaf 0); a[] is never needed because the leaf a new leaf
+//     was linked after is the stack's current child;
+//   * the gap computation of leaf l (mf.c:88-124) walks each dimension's
+//     chain m -> d[i][l][b[m]] and keeps the positions reached by every
+//     dimension that constrains l -- here as u64 masks instead of a gap stack
+//     and a per-leaf tally (positions are < l <= 64);
+//   * every legal position is one insert (node).  An insert at position 0 of
+//     a leaf l >= 2 breaks the normal-form filter b[0] == 1 (mf.c:83) and is
+//     pruned at once, so it is counted but never entered; an insert of leaf n
+//     is a completion worth +n (mf.c:84-85, process() :9-11);
+//   * b rows and the per-level mask stack are lane-interleaved in LDS; the
+//     d tables of every map in the launch sit in LDS as bytes.
+// Child order differs from the reference's gap-stack order; subtree counts,
+// inserts and completions do not depend on it.
+#include <hip/hip_runtime.h>
+
+#include "mf_device.hpp"
+#include "mf_wave.hpp"
+
+namespace mf {
+namespace {
+
+constexpr uint32_t kRowBytes = 4096;   // 16 words x 64 lanes: b[0..63]
+constexpr uint32_t kMapBytes = kMaxMaps * sizeof(LunnonMap);
+
+struct LLane {
+    uint32_t bl;      // LDS byte address of this lane's b row
+    uint32_t sl;      // LDS byte address of this lane's stack slot 0
+};
+
+struct LWork {
+    uint32_t l;       // next leaf to insert
+    uint32_t root;
+    uint32_t n;
+    uint32_t ndim;
+    uint32_t tab;     // LDS byte address of the map's d table
+    uint32_t base;    // leaf of stack slot 0
+    uint32_t origin;
+    uint32_t map;
+    uint64_t comps;   // completions of the current item
+    uint32_t inodes;  // inserts, gap computations, walk steps of the current item
+    uint32_t iwalks;
+    uint32_t isteps;
+    uint32_t icands;
+    bool have;
+    bool bad;         // a walk exceeded n steps: the state was not a folding state
+};
+
+struct LTotals {
+    unsigned long long sum = 0;   // sum of n * completions
+    unsigned long long nodes = 0, walks = 0, steps = 0, cands = 0, comps = 0, items = 0;
+};
+
+__device__ __forceinline__ uint64_t lstk_get(const uint8_t *lds, const LLane &ln, const LWork &w, uint32_t l) {
+    return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (l - w.base) * 512u);
+}
+__device__ __forceinline__ void lstk_put(uint8_t *lds, const LLane &ln, const LWork &w, uint32_t l, uint64_t m) {
+    *reinterpret_cast<uint64_t *>(lds + ln.sl + (l - w.base) * 512u) = m;
+}
+
+// Link leaf l directly below leaf m (mf.c:137-140 without a[]).
+__device__ __forceinline__ void link(uint8_t *lds, const LLane &ln, uint32_t m, uint32_t l) {
+    lds[row_addr(ln.bl, l)] = lds[row_addr(ln.bl, m)];
+    lds[row_addr(ln.bl, m)] = static_cast<uint8_t>(l);
+}
+// Unlink leaf l that sits below leaf m (mf.c:131-132).
+__device__ __forceinline__ void unlink(uint8_t *lds, const LLane &ln, uint32_t m, uint32_t l) {
+    lds[row_addr(ln.bl, m)] = lds[row_addr(ln.bl, l)];
+}
+
+// Legal positions for leaf l: mask of m < l reached by the walk of every
+// dimension with d[i][l][l] != l (all m < l if none constrains l).
+// A walk visits each position at most once, so it ends within l steps; the
+// guard only trips on a state that no folding DFS can produce.
+template <bool EV>
+__device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const LLane &ln, LWork &w) {
+    const uint32_t l = w.l, s = w.n + 1;
+    const uint64_t all = (l >= 64u) ? ~0ull : ((1ull << l) - 1ull);
+    uint64_t meet = all, reach_any = 0;
+    bool constrained = false;
+    for (uint32_t i = 0; i < w.ndim; ++i) {
+        const uint32_t row = w.tab + (i * s + l) * s;
+        uint32_t m = lds[row + l];
+        if (m == l) continue;
+        constrained = true;
+        uint64_t reach = 0;
+        uint32_t guard = 0;
+        do {
+            if (EV) ++w.isteps;
+            reach |= 1ull << m;
+            m = lds[row + lds[row_addr(ln.bl, m)]];
+            if (++guard > l) {
+                w.bad = true;
+                break;
+            }
+        } while (m != l);
+        meet &= reach;
+        reach_any |= reach;
+    }
+    if (EV) {
+        ++w.iwalks;
+        w.icands += constrained ? static_cast<uint32_t>(__popcll(reach_any)) : l;
+    }
+    return constrained ? meet : all;
+}
+
+__device__ __forceinline__ void lload_item(uint8_t *lds, const LLane &ln, const WorkItem *__restrict__ it,
+                                           const LunnonMap *maps, LWork &w) {
+    state_to_lds(lds, ln.bl, it->state);
+    const LunnonMap mp = maps[it->map];
+    w.n = mp.n;
+    w.ndim = mp.ndim;
+    w.tab = mp.table_off;
+    w.base = mp.base;
+    w.map = it->map;
+    w.origin = it->origin;
+    const uint32_t level = it->level;
+    const uint64_t mask = it->mask;
+    w.root = level;
+    w.comps = 0;
+    w.inodes = 0;
+    w.iwalks = 0;
+    w.isteps = 0;
+    w.icands = 0;
+    w.have = true;
+    w.bad = false;
+    if (mask == 0) {
+        w.l = level;
+    } else {
+        lstk_put(lds, ln, w, level, mask);
+        link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(mask)), level);
+        w.l = level + 1;
+    }
+}
+
+template <bool EV>
+__device__ __forceinline__ bool ldfs_step(uint8_t *lds, const LLane &ln, LWork &w) {
+    uint64_t g = gap_mask<EV>(lds, ln, w);
+    if (w.bad) return true;
+    w.inodes += static_cast<uint32_t>(__popcll(g));
+    g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
+    if (w.l == w.n) {
+        w.comps += static_cast<uint32_t>(__popcll(g));
+        g = 0;
+    }
+    if (g != 0) {
+        lstk_put(lds, ln, w, w.l, g);
+        link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(g)), w.l);
+        ++w.l;
+        return false;
+    }
+    for (;;) {
+        if (w.l == w.root) return true;
+        --w.l;
+        uint64_t m = lstk_get(lds, ln, w, w.l);
+        unlink(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.l);
+        m &= m - 1;
+        if (m != 0) {
+            lstk_put(lds, ln, w, w.l, m);
+            link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.l);
+            ++w.l;
+            return false;
+        }
+    }
+}
+
+__device__ __forceinline__ void lretire(const LunnonArgs &a, const LWork &w, LTotals &t) {
+    if (w.bad) atomicOr(a.acc + ACC_ERROR, 2ull);
+    const unsigned long long v = static_cast<unsigned long long>(w.n) * w.comps;
+    t.sum += v;
+    t.nodes += w.inodes;
+    t.walks += w.iwalks;
+    t.steps += w.isteps;
+    t.cands += w.icands;
+    t.comps += w.comps;
+    t.items += 1;
+    if (a.per_count != nullptr && w.comps != 0) atomicAdd(a.per_count + w.origin, static_cast<unsigned long long>(w.comps));
+    if (a.per_nodes != nullptr && w.inodes != 0) atomicAdd(a.per_nodes + w.origin, static_cast<unsigned long long>(w.inodes));
+    if (a.per_map != nullptr && v != 0) atomicAdd(a.per_map + w.map, v);
+}
+
+__device__ __forceinline__ void lflush(const LunnonArgs &a, uint32_t lane, const LTotals &t) {
+    add_u128(a.acc, t.sum);
+    const unsigned long long n = wave_sum(t.nodes), wk = wave_sum(t.walks), st = wave_sum(t.steps),
+                             cd = wave_sum(t.cands), cp = wave_sum(t.comps), it = wave_sum(t.items);
+    if (lane == 0) {
+        if (n) atomicAdd(a.acc + ACC_NODES, n);
+        if (wk) atomicAdd(a.acc + ACC_WALKS, wk);
+        if (st) atomicAdd(a.acc + ACC_STEPS, st);
+        if (cd) atomicAdd(a.acc + ACC_CANDS, cd);
+        if (cp) atomicAdd(a.acc + ACC_LEAVES, cp);
+        if (it) atomicAdd(a.acc + ACC_ITEMS, it);
+    }
+}
+
+// LDS: [tables][maps][per wave: b rows | stack].
+__device__ __forceinline__ uint32_t tables_span(uint32_t bytes) { return (bytes + 15u) & ~15u; }
+
+__device__ __forceinline__ LLane lmake_lane(uint32_t tab_span, uint32_t levels) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t wbase = tab_span + kMapBytes + wave * (kRowBytes + levels * 512u);
+    LLane ln;
+    ln.bl = wbase + lane * 4u;
+    ln.sl = wbase + kRowBytes + lane * 8u;
+    return ln;
+}
+
+__device__ __forceinline__ void lstage(const LunnonArgs &a, uint8_t *lds, uint32_t tab_span) {
+    // Tables (byte-granular copy, 4 bytes per thread and step) and map descriptors.
+    const uint32_t words = (a.tables_bytes + 3u) / 4u;
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) {
+        uint32_t v = 0;
+        for (uint32_t b = 0; b < 4u; ++b) {
+            const uint32_t at = 4u * k + b;
+            if (at < a.tables_bytes) v |= static_cast<uint32_t>(a.tables[at]) << (8u * b);
+        }
+        reinterpret_cast<uint32_t *>(lds)[k] = v;
+    }
+    LunnonMap *maps = reinterpret_cast<LunnonMap *>(lds + tab_span);
+    for (uint32_t k = threadIdx.x; k < a.num_maps; k += blockDim.x) maps[k] = a.maps[k];
+    __syncthreads();
+}
+
+}  // namespace
+
+template <bool EV>
+__global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t span = tables_span(a.tables_bytes);
+    lstage(a, lds, span);
+    const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
+    const LLane ln = lmake_lane(span, a.levels);
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    LTotals tot;
+    LWork w;
+    w.have = false;
+    if (idx < a.num_items) lload_item(lds, ln, a.items + idx, maps, w);
+    while (w.have) {
+        if (ldfs_step<EV>(lds, ln, w)) {
+            lretire(a, w, tot);
+            w.have = false;
+        }
+    }
+    lflush(a, threadIdx.x & 63u, tot);
+}
+
+template <bool EV>
+__global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t span = tables_span(a.tables_bytes);
+    lstage(a, lds, span);
+    const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
+    const LLane ln = lmake_lane(span, a.levels);
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    LTotals tot;
+    LWork w;
+    w.have = false;
+    bool drained = false;
+    uint32_t tail_left = a.tail;
+
+    for (;;) {
+        if (!drained) {
+            const unsigned long long idle = __ballot(!w.have);
+            if (idle != 0) {
+                const uint32_t want = static_cast<uint32_t>(__popcll(idle));
+                const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(idle));
+                uint32_t got = 0;
+                if (lane == first) got = atomicAdd(a.claim, want);
+                got = __shfl(got, first, 64);
+                if (got + want >= a.num_items) drained = true;
+                if (!w.have) {
+                    const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
+                    if (idx < a.num_items) lload_item(lds, ln, a.items + idx, maps, w);
+                }
+            }
+        }
+        if (!__any(w.have)) break;
+
+        for (uint32_t s = 0; s < a.budget && w.have; ++s) {
+            if (ldfs_step<EV>(lds, ln, w)) {
+                lretire(a, w, tot);
+                w.have = false;
+            }
+        }
+
+        if (drained) {
+            tail_left = tail_left > a.budget ? tail_left - a.budget : 0;
+            if (tail_left == 0) {
+                bool pend = w.have;
+                uint32_t t = w.l;
+                uint64_t mask = 0;
+                bool emit = pend;
+                while (__any(pend)) {
+                    uint32_t cnt = 0;
+                    const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
+                    if (emit) {
+                        if (slot < a.spill_cap) {
+                            WorkItem *dst = a.spill + slot;
+                            lds_to_state(lds, ln.bl, dst->state);
+                            dst->mask = mask;
+                            dst->origin = w.origin;
+                            dst->mult = static_cast<uint16_t>(w.n);
+                            dst->level = static_cast<uint8_t>(t);
+                            dst->map = static_cast<uint8_t>(w.map);
+                        } else {
+                            atomicOr(a.acc + ACC_ERROR, 1ull);
+                        }
+                    }
+                    if (cnt != 0 && lane == 0) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
+                    emit = false;
+                    if (pend) {
+                        if (t == w.root) {
+                            pend = false;
+                        } else {
+                            --t;
+                            const uint64_t m = lstk_get(lds, ln, w, t);
+                            unlink(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), t);
+                            mask = m & (m - 1);
+                            emit = mask != 0;
+                        }
+                    }
+                }
+                if (w.have) {
+                    lretire(a, w, tot);
+                    w.have = false;
+                }
+                break;
+            }
+        }
+    }
+    lflush(a, lane, tot);
+}
+
+void launch_lunnon_static(const LunnonArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
+                          hipStream_t s) {
+    if (events)
+        hipLaunchKernelGGL(lunnon_static_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL(lunnon_static_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+}
+
+void launch_lunnon_persistent(const LunnonArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
+                              hipStream_t s) {
+    if (events)
+        hipLaunchKernelGGL(lunnon_persistent_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
+    else
+        hipLaunchKernelGGL(lunnon_persistent_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+}
+
+uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block) {
+    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (kRowBytes + levels * 512u);
+}
+
+const void *lunnon_kernel_ptr(bool persistent, bool events) {
+    if (persistent) return events ? reinterpret_cast<const void *>(&lunnon_persistent_kernel<true>)
+                                  : reinterpret_cast<const void *>(&lunnon_persistent_kernel<false>);
+    return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<true>)
+                  : reinterpret_cast<const void *>(&lunnon_static_kernel<false>);
+}
+
+}  // namespace mf

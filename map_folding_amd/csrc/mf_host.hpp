@@ -1,0 +1,70 @@
+// mf_host.hpp -- host-side problem tables and frontier generators (C++17).
+//
+// These are the product's own generators (the CPU half of the hot path): the
+// Lunnon frontier collector and the symmetry-reduced seed / case stream.  They
+// define the work-item order that every count and per-item result refers to,
+// so their orders follow the reference exactly (cited per function).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mf {
+
+constexpr int kMaxLeaves = 64;   // n = prod(dims) <= 64 (one u64 mask per level)
+constexpr int kMaxDims = 4;      // GPU Lunnon kernel: up to 4 dimensions
+constexpr int kCells = 64;       // sym: up to 8x8 cells, one u64 mask
+
+// ------------------------------------------------------------------ Lunnon
+
+struct LunnonProblem {
+    int ndim = 0;
+    int n = 0;
+    int dims[kMaxDims] = {0, 0, 0, 0};
+    // d[i][l][m] for l, m in [0, n]; m = 0 is 0.  Stored as bytes: leaves <= 64.
+    std::vector<uint8_t> d;  // ndim * (n+1) * (n+1)
+    int stride() const { return n + 1; }
+    uint8_t at(int i, int l, int m) const { return d[(static_cast<size_t>(i) * (n + 1) + l) * (n + 1) + m]; }
+};
+
+// Returns an empty optional-like flag through `ok`; error text in `err`.
+bool lunnon_problem(const int32_t *dims, int ndim, LunnonProblem &pr, std::string &err);
+
+// Canonical frontier (collect_states order).  Appends b[0..D] rows to `out`
+// until `max_states` rows were written; returns the total frontier size.
+int64_t lunnon_frontier(const LunnonProblem &pr, int depth, uint8_t *out, int64_t max_states);
+
+// ------------------------------------------------------------------ sym
+
+struct SymProblem {
+    int n = 0;
+    int n2 = 0;
+    uint8_t edge[4][kCells];      // edge[p][c] = neighbour of cell c on page p, n2 if none
+    uint32_t edge_word[kCells];   // the 4 pages of cell c packed, page p in bits 8p..8p+7
+    int map[kCells];              // spiral number of (x, y) at map[x*n + y]
+};
+
+bool sym_problem(int n, SymProblem &pr, std::string &err);
+
+// Host get_good (used by the generators only).
+uint64_t sym_good(const SymProblem &pr, const uint8_t *cycle, int i);
+
+struct SymSeed {
+    uint8_t cycle[kCells];
+    uint64_t mult;
+};
+
+// Deduplicated center cycles in generation order; raw = number before dedup.
+std::vector<SymSeed> sym_seeds(const SymProblem &pr, int64_t *raw);
+
+// Per-seed case counts at level `expanded` (cached per (n, expanded) in the
+// process; computed on all host threads).
+const std::vector<uint64_t> &sym_seed_case_counts(const SymProblem &pr, int expanded);
+
+// Cases [start, start+max) in stream order.  Returns the number written.
+int64_t sym_cases(const SymProblem &pr, int expanded, uint64_t start, uint64_t max_cases,
+                  uint8_t *cycles, uint64_t *mults);
+
+}  // namespace mf

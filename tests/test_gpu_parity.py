@@ -1,0 +1,220 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle and the
+golden totals.  Integer results must match bit-exactly, including the exact
+event counters (nodes, walks, steps, leaves) that make up the roofline's
+algorithmic-work figure."""
+import collections
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "map_folding_amd", "bin")
+
+SCHEDULES = {
+    "static": dict(schedule=0),
+    "resplit": dict(schedule=1),
+    "resplit_tight": dict(schedule=1, budget=8, split_levels=1),
+}
+
+
+def _opts(mf, name, **kw):
+    d = dict(SCHEDULES[name])
+    d.update(kw)
+    return mf.SolveOptions(**d)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------------------------ Lunnon
+
+
+@pytest.mark.parametrize("sched", list(SCHEDULES))
+def test_lunnon_5x5_per_state_parity(gpu, oracle, golden, sched):
+    _say(f"lunnon 5x5 D15 per-state, schedule={sched}")
+    states = gpu.lunnon_frontier([5, 5], 15)
+    res, pc, pn = gpu.lunnon_solve([5, 5], 15, states, _opts(gpu, sched), per_item=True)
+    cnt, nodes, tot, st = oracle.lunnon_resume([5, 5], 15, states)
+    assert res.count == tot == golden["square_totals"]["5"]
+    assert np.array_equal(pc * 25, cnt)            # per-state completions x n
+    assert np.array_equal(pn, nodes)
+    assert res.nodes == st["inserts"]
+    assert res.walks == st["gapcomps"]
+    assert res.steps == st["walk_steps"]
+    assert res.cands == st["candidates"]
+    assert res.leaves == st["completions"]
+
+
+@pytest.mark.parametrize("dims,depth", [([4, 5], 9), ([3, 6], 7), ([2, 2, 3], 5), ([1, 8], 3), ([2, 3], 2),
+                                        ([2, 2, 2, 2], 6), ([4, 4], 1), ([3, 3], 7)])
+def test_lunnon_small_maps(gpu, oracle, dims, depth):
+    _say(f"lunnon {dims} D{depth}")
+    total, full = oracle.lunnon_count(dims)
+    for sched in ("static", "resplit_tight"):
+        count, nstates, res = gpu.gpu_folding(dims, depth, _opts(gpu, sched))
+        assert count == total, (dims, sched)
+        states, front = oracle.lunnon_frontier(dims, depth)
+        assert nstates == states.shape[0]
+        assert res.nodes + front["inserts"] == full["inserts"]
+        assert res.steps + front["walk_steps"] == full["walk_steps"]
+
+
+@pytest.mark.parametrize("sched", ["static", "resplit"])
+def test_lunnon_6x6_depth18_full(gpu, golden, sched):
+    _say(f"lunnon 6x6 D18 full, schedule={sched}")
+    count, nstates, res = gpu.gpu_folding([6, 6], 18, _opts(gpu, sched, count_events=False))
+    assert nstates == golden["lunnon_frontier_sizes"]["6x6@18"]
+    assert count == golden["square_totals"]["6"]
+
+
+def _rect_draw(seed, k):
+    """64 rectangles: p uniform in {2,3,4}, then q uniform with 12 <= p*q <= 30."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        p = int(rng.integers(2, 5))
+        qs = [q for q in range(1, 31) if 12 <= p * q <= 30]
+        out.append((p, int(rng.choice(qs))))
+    return out
+
+
+def test_lunnon_batch_small_rectangles(gpu, golden):
+    shapes = [s for s in _rect_draw(7, 64) if s[0] * s[1] <= 24][:40]
+    _say(f"lunnon batch {len(shapes)} maps")
+    depths = [max(1, p * q // 2) for p, q in shapes]
+    totals, res = gpu.lunnon_batch(shapes, depths, _opts(gpu, "resplit"))
+    for (p, q), t in zip(shapes, totals):
+        assert t == golden["rectangles"][f"{p}x{q}"], (p, q)
+    assert res.count == sum(totals)
+
+
+# ------------------------------------------------------------------------------ sym
+
+
+def _chunk(golden, n, e, start, mx):
+    for it in golden["sym_chunks"]["items"]:
+        if (it["n"], it["expanded"], it["start"], it["max"]) == (n, e, start, mx):
+            return it
+    raise KeyError((n, e, start, mx))
+
+
+@pytest.mark.parametrize("sched", ["resplit", "resplit_tight"])
+def test_sym_8x8_first_300_events(gpu, golden, sched):
+    _say(f"sym 8x8 d36 [0,300) schedule={sched}")
+    g = _chunk(golden, 8, 36, 0, 300)
+    cyc, mul = gpu.sym_cases(8, 36, 0, 300)
+    res, pc, pn = gpu.sym_solve(8, 36, cyc, mul, _opts(gpu, sched), per_item=True)
+    assert res.count * 64 == g["answer"]
+    assert res.nodes == g["nodes"]
+    assert res.walks == g["page_walks"]
+    assert res.steps == g["walk_steps"]
+    assert res.leaves == g["leaves"]
+    assert int(pn.sum()) == g["nodes"]
+    assert int((pc * mul).sum()) * 64 == g["answer"]
+
+
+def _small_cases(gpu, n, e, start, k, max_nodes):
+    """Cases of [start, start+k) whose subtree has at most max_nodes nodes (so
+    that one lane per case finishes quickly), found with the re-split schedule."""
+    cyc, mul = gpu.sym_cases(n, e, start, k)
+    _, _, pn = gpu.sym_solve(n, e, cyc, mul, _opts(gpu, "resplit"), per_item=True)
+    keep = pn <= max_nodes
+    return cyc[keep], mul[keep]
+
+
+def test_sym_per_case_parity_vs_oracle(gpu, oracle):
+    _say("sym per-case parity vs oracle (small 8x8 d36 and 7x7 d24 cases)")
+    for n, e, start, k in ((8, 36, 5000, 400), (7, 24, 1000, 400)):
+        cyc, mul = _small_cases(gpu, n, e, start, k, 400000)
+        assert cyc.shape[0] >= 20
+        cnt, nodes, tot, st = oracle.sym_solve(n, e, cyc, mul)
+        for sched in ("static", "resplit_tight"):
+            res, pc, pn = gpu.sym_solve(n, e, cyc, mul, _opts(gpu, sched), per_item=True)
+            assert np.array_equal(pc, cnt), sched
+            assert np.array_equal(pn, nodes), sched
+            assert res.count == tot
+            assert (res.nodes, res.walks, res.steps, res.leaves) == (
+                st["nodes"], st["page_walks"], st["walk_steps"], st["leaves"])
+
+
+def test_sym_headline_chunk_8x8(gpu, golden):
+    """The 50,000-case 8x8 depth-36 chunk at 120,000,000 (full size)."""
+    _say("sym 8x8 d36 [120000000,+50000) resplit")
+    g = _chunk(golden, 8, 36, 120000000, 50000)
+    cyc, mul = gpu.sym_cases(8, 36, 120000000, 50000)
+    assert cyc.shape[0] == 50000
+    res, _, _ = gpu.sym_solve(8, 36, cyc, mul, _opts(gpu, "resplit", count_events=False))
+    assert res.count * 64 == g["answer"]
+    g2 = _chunk(golden, 8, 36, 120000000, 200)
+    res2, _, _ = gpu.sym_solve(8, 36, cyc[:200], mul[:200], _opts(gpu, "resplit_tight"))
+    assert res2.count * 64 == g2["answer"]
+
+
+def test_sym_7x7_chunk(gpu, golden):
+    _say("sym 7x7 d24 [200000,+20000)")
+    g = _chunk(golden, 7, 24, 200000, 20000)
+    ans, res = gpu.sym_fold(7, 24, 200000, 20000)
+    assert ans == g["answer"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
+def test_sym_fold_small_squares(gpu, golden, n):
+    _say(f"sym_fold n={n}")
+    ans, _ = gpu.sym_fold(n)
+    assert ans == golden["square_totals"][str(n)]
+
+
+def test_sym_chunk_additivity_gpu(gpu):
+    _say("sym chunk additivity")
+    whole, _ = gpu.sym_fold(8, 36, 77777, 3000)
+    parts = sum(gpu.sym_fold(8, 36, 77777 + s, 1000)[0] for s in (0, 1000, 2000))
+    assert whole == parts
+
+
+def test_resident_frontier_repeat(gpu, golden):
+    _say("resident frontier solved twice")
+    g = _chunk(golden, 8, 36, 0, 2000)
+    cyc, mul = gpu.sym_cases(8, 36, 0, 2000)
+    f = gpu.SymFrontier(8, 36, cyc, mul)
+    r1, _, _ = f.solve(_opts(gpu, "resplit"))
+    order = np.arange(2000, dtype=np.uint32)[::-1].copy()
+    r2, _, _ = f.solve(_opts(gpu, "resplit_tight", order=order))
+    f.close()
+    assert r1.count * 64 == g["answer"] == r2.count * 64
+    assert r1.nodes == r2.nodes
+
+
+# ------------------------------------------------------------------------------ CLIs
+
+
+def _run(args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run(args, capture_output=True, text=True, env=e, timeout=300)
+
+
+def test_cli_gpu_folding_5x5(gpu):
+    _say("cli gpu_folding 5 15")
+    r = _run([os.path.join(BIN, "gpu_folding"), "5", "15"])
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == "Number of partial states at depth 15: 13416\nTotal number of ways: 186086600\n"
+    r = _run([os.path.join(BIN, "gpu_folding"), "5", "10"], {"MF_DIMS": "3x6"})
+    assert r.stdout.splitlines()[-1] == "Total number of ways: 2016432"
+
+
+def test_cli_sym_fold_chunk(gpu, golden):
+    _say("cli sym_fold 8 chunk")
+    g = _chunk(golden, 8, 36, 120000000, 200)
+    r = _run([os.path.join(BIN, "sym_fold"), "8", "16", "200", "120000000"],
+             {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "36"})
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == f"{g['answer']}\n"
+    r = _run([os.path.join(BIN, "sym_fold"), "6", "8"])
+    assert r.stdout == f"{golden['square_totals']['6']}\n"

@@ -1,0 +1,118 @@
+"""CPU tests of the product's host half: frontier generators, case stream,
+library exports and the drop-in CLIs' argument handling (no GPU compute)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "map_folding_amd", "bin")
+
+
+def test_library_exports_every_declared_symbol(mf):
+    lib = mf._abi.lib()
+    declared = mf.declared_symbols()
+    assert "mf_sym_solve" in declared and "mf_wasm_solve_cases" in declared
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_no_oracle_in_product():
+    """The product never loads the oracle (it is only the checker)."""
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "map_folding_amd")):
+        for fn in files:
+            if fn.endswith((".py", ".cpp", ".hpp", ".hip", ".h")) or fn == "Makefile":
+                text = open(os.path.join(dirpath, fn), errors="replace").read()
+                assert "oracle" not in text.replace("the oracle", ""), fn
+
+
+@pytest.mark.parametrize("dims,depth", [([5, 5], 15), ([4, 5], 9), ([6, 6], 18), ([3, 7], 10),
+                                        ([2, 3, 2], 6), ([1, 6], 3)])
+def test_lunnon_frontier_matches_oracle(mf, oracle, dims, depth):
+    ours = mf.lunnon_frontier(dims, depth)
+    ref, _ = oracle.lunnon_frontier(dims, depth)
+    assert ours.shape == ref.shape
+    assert np.array_equal(ours, ref)
+
+
+def test_lunnon_frontier_sizes_golden(mf, golden):
+    for key, value in golden["lunnon_frontier_sizes"].items():
+        if key.startswith("_"):
+            continue
+        shape, depth = key.split("@")
+        g = int(shape.split("x")[0])
+        assert mf.lunnon_frontier([g, g], int(depth)).shape[0] == value, key
+
+
+@pytest.mark.parametrize("n", [6, 7, 8])
+def test_sym_seeds_match_oracle(mf, oracle, n):
+    c1, m1, r1 = mf.sym_seeds(n)
+    c2, m2, r2 = oracle.sym_seeds(n)
+    assert r1 == r2
+    assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+
+
+@pytest.mark.parametrize("n,e,start,count", [(7, 24, 0, 500), (7, 24, 200000, 300), (8, 24, 414000, 2000),
+                                             (8, 28, 3000000, 700), (6, 16, 100, 50), (7, 30, 12345, 400)])
+def test_sym_case_stream_matches_oracle(mf, oracle, n, e, start, count):
+    c1, m1 = mf.sym_cases(n, e, start, count)
+    c2, m2 = oracle.sym_cases(n, e, start, count)
+    assert c1.shape == c2.shape
+    assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+
+
+def test_sym_case_totals_golden(mf, golden):
+    for key in ("7@24", "8@24", "8@26", "8@28", "8@30", "8@32"):
+        n, e = map(int, key.split("@"))
+        assert mf.sym_case_total(n, e) == golden["sym_case_totals"][key], key
+
+
+def test_sym_case_stream_end(mf):
+    total = mf.sym_case_total(7, 24)
+    cyc, mul = mf.sym_cases(7, 24, total - 5, 100)
+    assert cyc.shape[0] == 5
+    cyc, mul = mf.sym_cases(7, 24, total + 10, 100)
+    assert cyc.shape[0] == 0
+
+
+def test_bad_arguments_raise(mf):
+    with pytest.raises(mf.MFError):
+        mf.lunnon_frontier([9, 9], 10)     # 81 leaves > 64
+    with pytest.raises(mf.MFError):
+        mf.lunnon_frontier([5, 5], 0)
+    with pytest.raises(mf.MFError):
+        mf.sym_cases(8, 64, 0, 10)          # expanded = n^2 is rejected
+    with pytest.raises(mf.MFError):
+        mf.sym_cases(9, 24, 0, 10)
+
+
+def _run(args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    e["HIP_VISIBLE_DEVICES"] = e.get("HIP_VISIBLE_DEVICES", "")
+    return subprocess.run(args, capture_output=True, text=True, env=e, timeout=120)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "sym_fold")), reason="CLIs not built")
+def test_cli_argument_contract():
+    r = _run([os.path.join(BIN, "gpu_folding")])
+    assert r.returncode == 1 and "Usage" in r.stdout
+    r = _run([os.path.join(BIN, "gpu_folding"), "9", "10"])
+    assert r.returncode == 1 and "Grid too large" in r.stdout
+    r = _run([os.path.join(BIN, "sym_fold")])
+    assert r.returncode == 1
+    r = _run([os.path.join(BIN, "sym_fold"), "9"])
+    assert r.returncode == 1 and "between 1 and 8" in r.stderr
+    r = _run([os.path.join(BIN, "sym_fold"), "x"])
+    assert r.returncode == 2
+    r = _run([os.path.join(BIN, "sym_fold"), "2"])
+    assert r.returncode == 0 and r.stdout == "8\n"
+    r = _run([os.path.join(BIN, "sym_fold"), "1", "4"])
+    assert r.stdout == "1\n"
+    r = _run([os.path.join(BIN, "sym_fold"), "8", "1"], {"MF_SETUP_ONLY": "1"})
+    assert r.returncode == 0 and r.stdout == "414992\n"
+    assert "center_cases=9394" in r.stderr and "reduced_cases=1225" in r.stderr
+    r = _run([os.path.join(BIN, "sym_fold"), "8", "1"], {"MF_EXPANDED_N": "64"})
+    assert r.returncode == 1

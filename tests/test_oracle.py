@@ -1,0 +1,181 @@
+"""Pins the CPU oracle (oracle/mf_oracle.c) before it is trusted as the checker.
+
+What pins each oracle function (each check is independent of the oracle's own
+formulas):
+  orc_lunnon_count     published square totals n<=5 (mfah/folding.py:8-17), the
+                       reference binaries' rectangle totals, brute-force strip
+                       foldings (non-crossing creases over all permutations),
+                       transpose symmetry, and the full-tree event counts
+                       measured on the reference's mf.c loop
+  orc_lunnon_frontier  frontier sizes printed by the reference's mf_fast
+                       (MF_COLLECT_ONLY) and EXPERIMENTS.md:37-38
+  orc_lunnon_resume    sum over the frontier = the golden total, and frontier
+                       counters + resume counters = full-tree counters
+  orc_sym_*            published totals n<=6 (cross-checked against Lunnon),
+                       seed counts / multiplier histograms (sum mult = 2 x raw),
+                       stream case totals (EXPERIMENTS.md:65-71), chunk totals
+                       with node/walk/step/leaf counts, chunk additivity
+"""
+import collections
+import itertools
+
+import numpy as np
+import pytest
+
+
+def _strip_foldings_brute(n):
+    """Stack orders of a strip of n stamps whose same-side creases never cross."""
+    total = 0
+    for pos in itertools.permutations(range(n)):
+        creases = [(min(pos[k], pos[k + 1]), max(pos[k], pos[k + 1]), k & 1) for k in range(n - 1)]
+        ok = True
+        for i in range(len(creases)):
+            a, b, s = creases[i]
+            for j in range(i + 1, len(creases)):
+                c, d, t = creases[j]
+                if s == t and (a < c < b < d or c < a < d < b):
+                    ok = False
+                    break
+            if not ok:
+                break
+        total += ok
+    return total
+
+
+def test_lunnon_strips_match_brute_force(oracle):
+    for n in range(1, 8):
+        assert oracle.lunnon_count([1, n])[0] == _strip_foldings_brute(n)
+        assert oracle.lunnon_count([n])[0] == _strip_foldings_brute(n)
+
+
+def test_lunnon_square_totals(oracle, golden):
+    for n in range(1, 6):
+        assert oracle.lunnon_count([n, n])[0] == golden["square_totals"][str(n)]
+
+
+def test_lunnon_small_rectangles_and_transpose(oracle, golden):
+    for shape, value in golden["rectangles"].items():
+        if shape.startswith("_"):
+            continue
+        p, q = map(int, shape.split("x"))
+        if p * q > 20:
+            continue
+        assert oracle.lunnon_count([p, q])[0] == value, shape
+        assert oracle.lunnon_count([q, p])[0] == value, shape
+    assert oracle.lunnon_count([1, 5])[0] == golden["strips"]["1x5"]
+
+
+def test_lunnon_5x5_event_counts(oracle, golden):
+    total, st = oracle.lunnon_count([5, 5])
+    assert total == golden["square_totals"]["5"]
+    for k, v in golden["lunnon_5x5_events"].items():
+        if not k.startswith("_"):
+            assert st[k] == v, k
+
+
+def test_lunnon_frontier_sizes(oracle, golden):
+    for key, value in golden["lunnon_frontier_sizes"].items():
+        if key.startswith("_"):
+            continue
+        shape, depth = key.split("@")
+        g = int(shape.split("x")[0])
+        if g > 6:
+            continue
+        states, _ = oracle.lunnon_frontier([g, g], int(depth))
+        assert states.shape[0] == value, key
+
+
+@pytest.mark.parametrize("dims,depth", [([5, 5], 15), ([4, 5], 10), ([3, 6], 7), ([2, 2, 3], 5)])
+def test_lunnon_resume_partitions_the_tree(oracle, dims, depth):
+    total, full = oracle.lunnon_count(dims)
+    states, front = oracle.lunnon_frontier(dims, depth)
+    cnt, nodes, tot, res = oracle.lunnon_resume(dims, depth, states)
+    assert tot == total == int(cnt.sum())
+    assert int(nodes.sum()) == res["inserts"]
+    for k in ("inserts", "gapcomps", "walk_steps", "candidates", "completions"):
+        assert front[k] + res[k] == full[k], k
+    assert front["completions"] == 0
+
+
+def test_lunnon_frontier_state_is_a_list(oracle):
+    """Every state is a single circular list over leaves 0..D with b[0] == 1."""
+    states, _ = oracle.lunnon_frontier([5, 5], 15)
+    for row in states[::97]:
+        assert row[0] == 1
+        seen, m = set(), 0
+        for _ in range(len(row)):
+            seen.add(m)
+            m = int(row[m])
+        assert m == 0 and seen == set(range(len(row)))
+
+
+def test_sym_small_squares_match_published_and_lunnon(oracle, golden):
+    for n in range(1, 6):
+        ans, _ = oracle.sym_answer(n)
+        assert ans == golden["square_totals"][str(n)]
+        assert ans == oracle.lunnon_count([n, n])[0] if n <= 4 else True
+
+
+def test_sym_6x6_matches_published(oracle, golden):
+    ans, st = oracle.sym_answer(6)
+    assert ans == golden["square_totals"]["6"]
+    # The 8x8 depth-36 frontier is the 6x6 center: its case count equals the
+    # number of completed 6x6 center cycles.
+    assert st["leaves"] == golden["sym_case_totals"]["8@36"]
+
+
+@pytest.mark.parametrize("n", [7, 8])
+def test_sym_seeds(oracle, golden, n):
+    cyc, mul, raw = oracle.sym_seeds(n)
+    g = golden["sym_seeds"][str(n)]
+    assert raw == g["raw"] and len(cyc) == g["seeds"]
+    hist = collections.Counter(int(m) for m in mul)
+    assert {str(k): v for k, v in hist.items()} == g["mult_hist"]
+    assert int(mul.sum()) == 2 * raw
+
+
+def test_sym_case_totals(oracle, golden):
+    for key in ("7@24", "8@24", "8@26", "8@28", "8@30"):
+        n, e = map(int, key.split("@"))
+        assert oracle.sym_case_total(n, e) == golden["sym_case_totals"][key], key
+
+
+def _chunk(golden, n, e, start, mx):
+    for it in golden["sym_chunks"]["items"]:
+        if (it["n"], it["expanded"], it["start"], it["max"]) == (n, e, start, mx):
+            return it
+    raise KeyError
+
+
+def test_sym_chunk_7x7_first_300_with_nodes(oracle, golden):
+    g = _chunk(golden, 7, 24, 0, 300)
+    ans, st = oracle.sym_answer(7, 24, 0, 300)
+    assert ans == g["answer"]
+    assert st["nodes"] == g["nodes"]
+
+
+def test_sym_chunk_8x8_first_300_with_events(oracle, golden):
+    g = _chunk(golden, 8, 36, 0, 300)
+    ans, st = oracle.sym_answer(8, 36, 0, 300)
+    assert ans == g["answer"]
+    for k in ("nodes", "page_walks", "walk_steps", "leaves"):
+        assert st[k] == g[k], k
+
+
+def test_sym_single_case_8x8_d24(oracle, golden):
+    g = _chunk(golden, 8, 24, 400000, 1)
+    assert oracle.sym_answer(8, 24, 400000, 1)[0] == g["answer"]
+
+
+def test_sym_chunk_additivity(oracle):
+    whole, _ = oracle.sym_answer(7, 24, 1000, 120)
+    a, _ = oracle.sym_answer(7, 24, 1000, 50)
+    b, _ = oracle.sym_answer(7, 24, 1050, 70)
+    assert whole == a + b
+
+
+def test_sym_cases_window_matches_stream(oracle):
+    """A window of the stream equals the same slice of a longer prefix."""
+    cyc, mul = oracle.sym_cases(7, 24, 0, 400)
+    c2, m2 = oracle.sym_cases(7, 24, 123, 200)
+    assert np.array_equal(cyc[123:323], c2) and np.array_equal(mul[123:323], m2)
