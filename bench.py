@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark: symmetry-reduced map-folding DFS on one MI355X.
+
+Workload: the 8x8 depth-36 expanded-case chunk [start, start+50000) of the
+sym_fold case stream (MF_EXPANDED_N=36), default start 120,000,000.  A step is
+one pass of the hot path over the chunk: every case resumed and counted to
+completion on the GPU, cases resident in HBM before the timed region.  The
+chunk total is checked bit-exactly (against the published/reference golden
+for the default chunk) on every timed step.
+
+Prints ONE JSON line.  `value` is DFS nodes/s (node = one get_good call) of
+the batched persistent re-split schedule; the plain static-frontier kernel
+(one lane per case, one launch) is timed on the same chunk beside it.
+
+    python bench.py [--steps K] [--warmup W] [--start S] [--cases C]
+                    [--no-plain] [--no-cpu] [--cpu-cases M]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+GOLDEN_CHUNK = (120000000, 50000, 26687875796992)   # published/reference chunk total (x n^2)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--start", type=int, default=GOLDEN_CHUNK[0])
+    ap.add_argument("--cases", type=int, default=GOLDEN_CHUNK[1])
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain static-frontier kernel")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cases", type=int, default=4000)
+    ap.add_argument("--budget", type=int, default=0)
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this workload has no multi-GPU path in this build "
+              "(chunks shard as independent case ranges with one 8-byte sum; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import map_folding_amd as mf
+    from map_folding_amd import roofline as rl
+
+    n, level = 8, 36
+    mf.init(0)
+    cus = mf.device_cus()
+    t0 = time.time()
+    cyc, mul = mf.sym_cases(n, level, args.start, args.cases)
+    gen_s = time.time() - t0
+    log(f"frontier: {cyc.shape[0]} cases from {args.start} generated in {gen_s:.2f}s (host)")
+    front = mf.SymFrontier(n, level, cyc, mul)
+
+    batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
+                              waves_per_cu=args.waves_per_cu)
+    # Exact event counts (schedule-independent) for the algorithmic-work figure.
+    res, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True))
+    nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
+    total = res.count * n * n
+    golden = GOLDEN_CHUNK[2] if (args.start, args.cases) == GOLDEN_CHUNK[:2] else None
+    if golden is not None and total != golden:
+        raise SystemExit(f"chunk total {total} != golden {golden}")
+    log(f"counting run: total={total} nodes={nodes} walks={walks} steps={steps} rounds={res.rounds} "
+        f"kernel_ms={res.kernel_ms:.1f}")
+
+    for w in range(args.warmup):
+        r, _, _ = front.solve(batched)
+        log(f"warmup {w}: {r.total_ms:.1f} ms")
+
+    wall, kern, span, rounds = [], [], [], []
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        r, _, _ = front.solve(batched)
+        wall.append(time.perf_counter() - t1)
+        if r.count * n * n != total or r.nodes != nodes:
+            raise SystemExit("a timed step disagreed with the counting run")
+        kern.append(r.kernel_ms / 1e3)
+        span.append(r.span_ms / 1e3)
+        rounds.append(r.rounds)
+        log(f"step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}")
+
+    ms_per_step = 1e3 * sum(wall) / len(wall)
+    node_rate = nodes / (ms_per_step / 1e3)
+    case_rate = cyc.shape[0] / (ms_per_step / 1e3)
+    kernel_s = sum(kern) / len(kern)
+    ops = rl.sym_ops(nodes, walks, steps)
+    achieved = ops / kernel_s / 1e12
+    peak = rl.int32_peak_ops(cus or 256) / 1e12
+
+    plain = None
+    if not args.no_plain:
+        t1 = time.perf_counter()
+        rp, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_STATIC, count_events=False))
+        pw = time.perf_counter() - t1
+        if rp.count * n * n != total:
+            raise SystemExit("static schedule disagreed")
+        plain = {"nodes_per_s": nodes / pw, "cases_per_s": cyc.shape[0] / pw, "ms": pw * 1e3,
+                 "kernel_ms": rp.kernel_ms, "frac_of_int32_peak": ops / (rp.kernel_ms / 1e3) / 1e12 / peak}
+        log(f"plain static: {pw*1e3:.1f} ms")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        m = min(args.cpu_cases, cyc.shape[0])
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        _, _, csum, cst = oracle.sym_solve(n, level, cyc[:m], mul[:m], threads=threads)
+        cs = time.perf_counter() - t1
+        cpu = {"value": cst["nodes"] / cs, "unit": "nodes/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} cases of the chunk ({cst['nodes']} nodes), oracle C restatement of "
+                         f"sym_fold.cpp recurse_count on {threads} threads, {cs:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3e} nodes/s on {threads} threads ({cs:.1f}s)")
+
+    out = {
+        "metric": "DFS nodes/s & cases/s on 1 MI355X, batched+learned vs plain, bit-exact",
+        "value": node_rate,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: deterministic sym_fold case stream (no dataset)",
+        "config": {"workload": f"8x8 sym_fold MF_EXPANDED_N=36 chunk [{args.start},{args.start + cyc.shape[0]})",
+                   "cases": int(cyc.shape[0]), "schedule": "persistent re-split", "chunk_total": str(total),
+                   "bit_exact_vs_golden": golden is not None},
+        "cases_per_s": case_rate,
+        "nodes": nodes, "page_walks": walks, "walk_steps": steps, "leaves": leaves,
+        "rounds": rounds[-1] if rounds else None,
+        "kernel_ms_per_step": kernel_s * 1e3,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "ops_per_step": ops, "ops_model": "16*nodes + 6*page_walks + 8*walk_steps (int32)"},
+        "plain": plain,
+        "speedup_vs_plain": (node_rate / plain["nodes_per_s"]) if plain else None,
+        "cpu_baseline": cpu,
+        "frontier_gen_s": gen_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

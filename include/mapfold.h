@@ -70,9 +70,18 @@ typedef struct {
     uint64_t items;        /* work items executed (>= input items when re-split) */
     uint32_t rounds;       /* kernel rounds (1 for the static schedule) */
     uint32_t status;
-    double kernel_ms;      /* device time of the solve kernels (HIP events) */
+    double kernel_ms;      /* sum of the solve kernels' durations (HIP events per launch) */
+    double span_ms;        /* device time from the first launch to the end of the last */
     double total_ms;       /* host wall time of the call */
 } mf_result;
+
+/* Frozen int32-op weights of the algorithmic work model (roofline figure):
+ * sym ops = 16*nodes + 6*walks + 8*steps;
+ * Lunnon ops = 10*gapcomps + 8*steps + 6*cands + 8*inserts + 2*completions. */
+enum {
+    MF_W_SYM_NODE = 16, MF_W_SYM_WALK = 6, MF_W_SYM_STEP = 8,
+    MF_W_LUN_GAPCOMP = 10, MF_W_LUN_STEP = 8, MF_W_LUN_CAND = 6, MF_W_LUN_INSERT = 8, MF_W_LUN_COMPLETION = 2
+};
 
 /* Scheduling options.  Zero-initialise and set what you need; zeros select
  * tuned defaults.  Scheduling never changes any count. */

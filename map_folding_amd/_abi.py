@@ -31,7 +31,7 @@ class MFResult(C.Structure):
         ("nodes", C.c_uint64), ("walks", C.c_uint64), ("steps", C.c_uint64),
         ("leaves", C.c_uint64), ("cands", C.c_uint64), ("items", C.c_uint64),
         ("rounds", C.c_uint32), ("status", C.c_uint32),
-        ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double), ("span_ms", C.c_double), ("total_ms", C.c_double),
     ]
 
     @property

@@ -76,7 +76,8 @@ struct Context {
     int device = -1;
     int cus = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // span of a solve
+    hipEvent_t la = nullptr, lb = nullptr;     // one launch
     DevBuf items, spill_a, spill_b, acc, counters, per_count, per_nodes, per_map, tables;
     unsigned long long *host_acc = nullptr;   // pinned
     uint32_t *host_counters = nullptr;        // pinned: [claim, spill_count]
@@ -185,6 +186,7 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
     args.spill_count = spill_count;
 
     uint32_t rounds = 0;
+    double launch_ms = 0.0;
     if (num > 0) {
         if (t.schedule == MF_SCHED_STATIC) {
             if ((rc = set_lds_limit(kstatic, lds_bytes)) != MF_OK) return rc;
@@ -216,17 +218,22 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
             int next = 0;
             HIP_TRY(hipEventRecord(g.ev0, g.stream));
             while (in_num > 0) {
+                float one = 0.0f;
                 if (rounds >= 100000) return fail(MF_ERR_INTERNAL, "re-split did not converge");
                 HIP_TRY(hipMemsetAsync(claim, 0, 2 * sizeof(uint32_t), g.stream));
                 args.items = in;
                 args.num_items = static_cast<uint32_t>(in_num);
                 args.spill = bufs[next];
                 args.spill_cap = static_cast<uint32_t>(cap);
+                HIP_TRY(hipEventRecord(g.la, g.stream));
                 launch_p(args, grid, t.block, lds_bytes, t.events, g.stream);
                 HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(g.lb, g.stream));
                 HIP_TRY(hipMemcpyAsync(g.host_counters, claim, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
                 HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, args.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost, g.stream));
                 HIP_TRY(hipStreamSynchronize(g.stream));
+                HIP_TRY(hipEventElapsedTime(&one, g.la, g.lb));
+                launch_ms += one;
                 ++rounds;
                 if (g.host_acc[ACC_ERROR] & 2ull) return fail(MF_ERR_INTERNAL, "a state walk did not close");
                 if (g.host_acc[ACC_ERROR] != 0)
@@ -252,7 +259,8 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
     if (num > 0) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
-        out->kernel_ms = ms;
+        out->span_ms = ms;
+        out->kernel_ms = (t.schedule == MF_SCHED_STATIC) ? ms : launch_ms;
     }
     out->total_ms = ms_since(t0);
     return MF_OK;
@@ -447,6 +455,8 @@ int mf_init(int device) {
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&g.ev0));
     HIP_TRY(hipEventCreate(&g.ev1));
+    HIP_TRY(hipEventCreate(&g.la));
+    HIP_TRY(hipEventCreate(&g.lb));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_acc), ACC_SLOTS * sizeof(unsigned long long)));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_counters), 4 * sizeof(uint32_t)));
     g.device = device;
@@ -466,6 +476,8 @@ void mf_shutdown(void) {
     if (g.host_counters) (void)hipHostFree(g.host_counters);
     if (g.ev0) (void)hipEventDestroy(g.ev0);
     if (g.ev1) (void)hipEventDestroy(g.ev1);
+    if (g.la) (void)hipEventDestroy(g.la);
+    if (g.lb) (void)hipEventDestroy(g.lb);
     if (g.stream) (void)hipStreamDestroy(g.stream);
     g = Context();
 }
@@ -723,6 +735,7 @@ int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
             out->items += part.items;
             out->rounds += part.rounds;
             out->kernel_ms += part.kernel_ms;
+            out->span_ms += part.span_ms;
             pos += static_cast<uint64_t>(got);
             left -= static_cast<uint64_t>(got);
             if (static_cast<uint64_t>(got) < want) break;

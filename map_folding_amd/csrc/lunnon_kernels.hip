@@ -279,6 +279,8 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
                     if (idx < a.num_items) lload_item(lds, ln, a.items + idx, maps, w);
                 }
+            } else {
+                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.num_items;
             }
         }
         if (!__any(w.have)) break;

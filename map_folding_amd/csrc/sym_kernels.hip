@@ -250,6 +250,10 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
                     if (idx < a.num_items) load_item(lds, ln, a.items + idx, w);
                 }
+            } else {
+                // Every lane is busy: look at the shared cursor anyway, so that
+                // a wave of long items still learns that the input is gone.
+                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.num_items;
             }
         }
         if (!__any(w.have)) break;
