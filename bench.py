@@ -42,7 +42,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cases", type=int, default=4000)
     ap.add_argument("--budget", type=int, default=0)
+    ap.add_argument("--tail", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--core", type=int, default=0,
+                    help="sym DFS stack: 0 default, 1 one u64 mask per level, 2 compact 16-bit entries")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -66,9 +69,9 @@ def main():
     front = mf.SymFrontier(n, level, cyc, mul)
 
     batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
-                              waves_per_cu=args.waves_per_cu)
+                              tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
     # Exact event counts (schedule-independent) for the algorithmic-work figure.
-    res, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True))
+    res, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core))
     nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
     total = res.count * n * n
     golden = GOLDEN_CHUNK[2] if (args.start, args.cases) == GOLDEN_CHUNK[:2] else None
@@ -104,7 +107,7 @@ def main():
     plain = None
     if not args.no_plain:
         t1 = time.perf_counter()
-        rp, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_STATIC, count_events=False))
+        rp, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_STATIC, count_events=False, core=args.core))
         pw = time.perf_counter() - t1
         if rp.count * n * n != total:
             raise SystemExit("static schedule disagreed")

@@ -91,9 +91,13 @@ typedef struct {
                                  MF_SCHED_RESPLIT: persistent lanes + on-device re-split */
     int32_t block;            /* threads per workgroup (64..256) */
     int32_t waves_per_cu;     /* persistent schedule: resident waves per CU */
-    int32_t budget;           /* re-split: nodes a lane runs on one item before it may split */
-    int32_t split_levels;     /* re-split: at most this many levels below an item's root donate */
+    int32_t budget;           /* re-split: DFS iterations between a wave's claim / drain checks */
+    int32_t tail;             /* re-split: iterations a lane may still run once the input is
+                                 drained before it splits its remaining work */
     int32_t count_events;     /* 1: count walks/steps too (nodes/leaves always counted) */
+    int32_t core;             /* sym DFS stack: 0 = default (compact), 1 = one u64 mask per
+                                 level, 2 = compact 16-bit entries with recompute */
+    int32_t reserved;
     uint64_t queue_capacity;  /* re-split: work items the device queue can hold */
     const uint32_t *order;    /* optional host permutation of item indices (learned order);
                                  NULL = input order.  Never changes any count. */

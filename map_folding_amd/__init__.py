@@ -47,8 +47,9 @@ class SolveOptions:
     block: int = 0
     waves_per_cu: int = 0
     budget: int = 0
-    split_levels: int = 0
+    tail: int = 0
     count_events: bool = True
+    core: int = 0                     # sym stack: 0 default, 1 wide u64 masks, 2 compact 16-bit
     queue_capacity: int = 0
     order: np.ndarray | None = None   # permutation of item indices (learned order)
 
@@ -58,8 +59,9 @@ class SolveOptions:
         o.block = int(self.block)
         o.waves_per_cu = int(self.waves_per_cu)
         o.budget = int(self.budget)
-        o.split_levels = int(self.split_levels)
+        o.tail = int(self.tail)
         o.count_events = 1 if self.count_events else 0
+        o.core = int(self.core)
         o.queue_capacity = int(self.queue_capacity)
         keep = None
         if self.order is not None:

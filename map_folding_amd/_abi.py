@@ -47,7 +47,8 @@ class MFResult(C.Structure):
 class MFSolveOptions(C.Structure):
     _fields_ = [
         ("schedule", C.c_int32), ("block", C.c_int32), ("waves_per_cu", C.c_int32),
-        ("budget", C.c_int32), ("split_levels", C.c_int32), ("count_events", C.c_int32),
+        ("budget", C.c_int32), ("tail", C.c_int32), ("count_events", C.c_int32),
+        ("core", C.c_int32), ("reserved", C.c_int32),
         ("queue_capacity", C.c_uint64), ("order", C.POINTER(C.c_uint32)),
     ]
 

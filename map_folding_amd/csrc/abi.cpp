@@ -19,12 +19,12 @@
 #include "mf_host.hpp"
 
 namespace mf {
-void launch_sym_static(const SymArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
-void launch_sym_persistent(const SymArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
-uint32_t sym_lds_bytes(uint32_t levels, uint32_t block);
-const void *sym_kernel_ptr(bool persistent, bool events);
-void launch_lunnon_static(const LunnonArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
-void launch_lunnon_persistent(const LunnonArgs &, uint32_t, uint32_t, uint32_t, bool, hipStream_t);
+hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid, uint32_t block, uint32_t lds,
+                      bool events, hipStream_t);
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block);
+const void *sym_kernel_ptr(int core, bool persistent, bool events);
+hipError_t launch_lunnon(const LunnonArgs &, bool persistent, uint32_t grid, uint32_t block, uint32_t lds,
+                         bool events, hipStream_t);
 uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block);
 const void *lunnon_kernel_ptr(bool persistent, bool events);
 }  // namespace mf
@@ -99,6 +99,7 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 
 struct Tuned {
     int schedule;
+    int core;
     uint32_t block;
     uint32_t waves_per_cu;
     uint32_t budget;
@@ -107,14 +108,18 @@ struct Tuned {
     uint64_t capacity;
 };
 
+constexpr int kDefaultCore = 2;   // compact 16-bit stack
+
 Tuned tune(const mf_solve_options *opt) {
     Tuned t;
     t.schedule = opt ? opt->schedule : MF_SCHED_RESPLIT;
+    t.core = (opt && (opt->core == 1 || opt->core == 2)) ? opt->core : kDefaultCore;
     t.block = (opt && opt->block > 0) ? static_cast<uint32_t>(opt->block) : 256u;
     t.block = std::max(64u, std::min(256u, (t.block / 64u) * 64u));
     t.waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
+    // A step expands one node (or recomputes one).
     t.budget = (opt && opt->budget > 0) ? static_cast<uint32_t>(opt->budget) : 64u;
-    t.tail = (opt && opt->split_levels > 0) ? static_cast<uint32_t>(opt->split_levels) * t.budget : t.budget;
+    t.tail = (opt && opt->tail > 0) ? static_cast<uint32_t>(opt->tail) : t.budget;
     t.events = opt ? opt->count_events != 0 : true;
     t.capacity = (opt && opt->queue_capacity > 0) ? opt->queue_capacity : 0;
     return t;
@@ -148,11 +153,11 @@ void fill_result(mf_result *out, const unsigned long long *acc) {
 }
 
 // Generic solve driver over a device-resident item array.
-//   Launch(args_items, num, spill_target, grid, persistent) runs one launch.
-template <typename Args, typename LaunchS, typename LaunchP>
+//   launch(args, persistent, grid) runs one kernel launch on g.stream.
+template <typename Args, typename Launch>
 int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds_bytes, const Tuned &t,
               const WorkItem *dev_items, uint64_t num, uint64_t origins, uint64_t *per_count, uint64_t *per_nodes,
-              uint64_t *per_map, uint32_t nmaps, mf_result *out, LaunchS launch_s, LaunchP launch_p) {
+              uint64_t *per_map, uint32_t nmaps, mf_result *out, Launch launch) {
     const auto t0 = std::chrono::steady_clock::now();
     std::memset(out, 0, sizeof(*out));
     int rc;
@@ -196,8 +201,7 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
             args.spill_cap = 0;
             const uint32_t grid = static_cast<uint32_t>((num + t.block - 1) / t.block);
             HIP_TRY(hipEventRecord(g.ev0, g.stream));
-            launch_s(args, grid, t.block, lds_bytes, t.events, g.stream);
-            HIP_TRY(hipGetLastError());
+            HIP_TRY(launch(args, false, grid));
             HIP_TRY(hipEventRecord(g.ev1, g.stream));
             rounds = 1;
         } else {
@@ -226,8 +230,7 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
                 args.spill = bufs[next];
                 args.spill_cap = static_cast<uint32_t>(cap);
                 HIP_TRY(hipEventRecord(g.la, g.stream));
-                launch_p(args, grid, t.block, lds_bytes, t.events, g.stream);
-                HIP_TRY(hipGetLastError());
+                HIP_TRY(launch(args, true, grid));
                 HIP_TRY(hipEventRecord(g.lb, g.stream));
                 HIP_TRY(hipMemcpyAsync(g.host_counters, claim, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
                 HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, args.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost, g.stream));
@@ -354,9 +357,12 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         SymArgs a;
         uint32_t levels = 0;
         sym_args(*f, a, &levels);
-        const uint32_t lds = sym_lds_bytes(levels, t.block);
-        rc = run_solve(a, sym_kernel_ptr(false, t.events), sym_kernel_ptr(true, t.events), lds, t, dev_items, f->num,
-                       f->num, per_count, per_nodes, nullptr, 0, out, launch_sym_static, launch_sym_persistent);
+        const uint32_t lds = sym_lds_bytes(t.core, levels, t.block);
+        auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
+            return launch_sym(args, t.core, persistent, grid, t.block, lds, t.events, g.stream);
+        };
+        rc = run_solve(a, sym_kernel_ptr(t.core, false, t.events), sym_kernel_ptr(t.core, true, t.events), lds, t,
+                       dev_items, f->num, f->num, per_count, per_nodes, nullptr, 0, out, launch);
     } else {
         LunnonArgs a;
         std::vector<uint8_t> blob;
@@ -371,9 +377,12 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
             if ((rc = g.per_map.ensure(f->maps.size() * 8)) != MF_OK) return rc;
             a.per_map = static_cast<unsigned long long *>(g.per_map.ptr);
         }
+        auto launch = [&](const LunnonArgs &args, bool persistent, uint32_t grid) {
+            return launch_lunnon(args, persistent, grid, t.block, lds, t.events, g.stream);
+        };
         rc = run_solve(a, lunnon_kernel_ptr(false, t.events), lunnon_kernel_ptr(true, t.events), lds, t, dev_items,
                        f->num, f->num, per_count, per_nodes, per_map, static_cast<uint32_t>(f->maps.size()), out,
-                       launch_lunnon_static, launch_lunnon_persistent);
+                       launch);
     }
     reordered.release();
     return rc;

@@ -340,22 +340,6 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     lflush(a, lane, tot);
 }
 
-void launch_lunnon_static(const LunnonArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
-                          hipStream_t s) {
-    if (events)
-        hipLaunchKernelGGL(lunnon_static_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL(lunnon_static_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
-}
-
-void launch_lunnon_persistent(const LunnonArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
-                              hipStream_t s) {
-    if (events)
-        hipLaunchKernelGGL(lunnon_persistent_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL(lunnon_persistent_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
-}
-
 uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block) {
     return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (kRowBytes + levels * 512u);
 }
@@ -365,6 +349,12 @@ const void *lunnon_kernel_ptr(bool persistent, bool events) {
                                   : reinterpret_cast<const void *>(&lunnon_persistent_kernel<false>);
     return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<true>)
                   : reinterpret_cast<const void *>(&lunnon_static_kernel<false>);
+}
+
+hipError_t launch_lunnon(const LunnonArgs &a, bool persistent, uint32_t grid, uint32_t block, uint32_t lds_bytes,
+                         bool events, hipStream_t s) {
+    void *args[] = {const_cast<LunnonArgs *>(&a)};
+    return hipLaunchKernel(lunnon_kernel_ptr(persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
 }
 
 }  // namespace mf

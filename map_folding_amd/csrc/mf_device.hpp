@@ -52,13 +52,13 @@ struct LunnonMap {
 constexpr int kMaxMaps = 64;
 
 struct SymArgs {
-    uint32_t edge_word[64];     // 4 pages of cell c packed into one word
+    uint32_t edge_word[64];     // 4 pages of cell c packed into one word (partner, n2 = none)
     const WorkItem *items;
     uint32_t num_items;
     uint32_t last;              // n2 - 1: the level whose good mask is counted
     uint32_t base_level;        // lowest level any item starts at (stack slot 0)
     uint32_t levels;            // stack slots per lane
-    uint32_t budget;            // persistent: node steps between claim/spill checks
+    uint32_t budget;            // persistent: core steps between claim/spill checks
     uint32_t tail;              // persistent: steps a lane may run after the input is drained
     uint32_t *claim;            // persistent: next input index
     WorkItem *spill;            // persistent: output queue of split work

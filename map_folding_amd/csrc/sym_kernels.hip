@@ -3,24 +3,40 @@
 // Restates recurse_count / get_good / iter_good (sym_fold.cpp:179-233) as a
 // per-lane iterative DFS:
 //   * one lane owns one subtree (a WorkItem);
-//   * its cycle (cell -> successor, bytes) and a stack of remaining-children
-//     masks (one u64 per level) live in LDS, lane-interleaved so that lanes
-//     never share a bank: cycle dword w of lane L is at word (w*64 + L), stack
-//     slot s of lane L at u64 (s*64 + L);
-//   * the four edge pages are packed one word per cell in LDS (page p in byte p);
+//   * its cycle (cell -> successor, bytes) lives in LDS, lane-interleaved so
+//     that lanes never share a bank (dword w of lane L at word w*64 + L);
+//   * the partner of every cell on each of the 4 pages sits in an LDS table
+//     (byte p of word c);
 //   * node = one get_good call (sym_fold9.cpp:342-344); a node at the last
 //     level adds popcount(good) leaves instead of recursing.
-// Two schedules share this core:
-//   sym_static_kernel      one lane per input item, one launch (the plain
-//                          static-frontier schedule);
-//   sym_persistent_kernel  resident lanes claim items from the input array a
-//                          wave at a time; once the input is drained, a lane
-//                          that is still busy after `tail` more steps splits
-//                          its remaining work (the node it is about to expand,
-//                          plus the untried siblings of every level on its
-//                          stack) into the spill queue, which the host feeds
-//                          back as the next round's input.
-// Counts never depend on the schedule: every subtree is counted exactly once.
+//
+// The DFS stack holds, per level, the children of that level's node not yet
+// entered.  Two encodings:
+//   compact (default)  16 bits per level: the current child j, the next
+//                      sibling, and whether more siblings follow.  When a
+//                      level's known siblings run out but more follow, the
+//                      node's good mask is recomputed (uncounted) and masked
+//                      to the children above j.  Nodes with three or more
+//                      children are ~10% of the tree at 8x8 depth 36, so the
+//                      recomputes cost little, while the stack shrinks from
+//                      8 to 2 bytes per level -- 118 instead of 280 bytes of
+//                      LDS per lane, 20 instead of 8 resident waves per CU.
+//   wide               one u64 mask per level (kept for comparison).
+// Walk semantics: for each page with an earlier neighbour (the anchor), get_good
+// follows cycle successors from the anchor; a cell whose page partner is
+// already placed is replaced by that partner; every cell reached is a legal
+// insertion point; the page masks are intersected.  Each page's walk loop
+// runs until every lane of the wave has closed its walk.
+//
+// Two schedules share the core:
+//   static      one lane per input item, one launch (the plain static-frontier
+//               schedule);
+//   persistent  resident lanes claim input items a wave at a time; once the
+//               input is drained, a lane still busy after `tail` iterations
+//               splits its remaining work -- the node it is about to expand
+//               plus the untried siblings of every level on its stack -- into
+//               the spill queue, which is the next round's input.
+// Counts never depend on encoding or schedule: every subtree is counted once.
 #include <hip/hip_runtime.h>
 
 #include "mf_device.hpp"
@@ -29,52 +45,19 @@
 namespace mf {
 namespace {
 
-constexpr uint32_t kEdgeBytes = 256;
+constexpr uint32_t kPartnerOff = 0;      // 64 words: byte p = partner on page p (n2 = none)
+constexpr uint32_t kTableBytes = 256;
 constexpr uint32_t kCycleBytes = 4096;   // 16 words x 64 lanes
 
 __device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_addr(lane_base, cell); }
 
-// get_good for cell i against this lane's cycle.  Walk semantics of
-// sym_fold.cpp:188-195: from the anchor (cell i's neighbour on page p), follow
-// cycle successors; a cell whose page-p partner is already placed is replaced
-// by that partner; every cell reached is a legal insertion point.
-template <bool EV>
-__device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
-                                              uint32_t &walks, uint32_t &steps) {
-    uint64_t good = (i >= 64u) ? ~0ull : ((1ull << i) - 1ull);
-    const uint32_t ew = reinterpret_cast<const uint32_t *>(lds)[i];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const uint32_t anchor = (ew >> (8 * p)) & 0xffu;
-        if (anchor < i && good != 0) {
-            if (EV) ++walks;
-            uint64_t reach = 1ull << anchor;
-            uint32_t j = lds[cyc_addr(cl, anchor)];
-            while (j != anchor) {
-                if (EV) ++steps;
-                const uint32_t partner = lds[j * 4u + p];
-                j = (partner < i) ? partner : j;
-                reach |= 1ull << j;
-                j = lds[cyc_addr(cl, j)];
-            }
-            good &= reach;
-        }
-    }
-    return good;
-}
+__device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ? ~0ull : ((1ull << i) - 1ull); }
 
 struct Lane {
     uint32_t cl;      // LDS byte address of this lane's cycle (dword 0)
     uint32_t sl;      // LDS byte address of this lane's stack slot 0
     uint32_t base;    // level held by stack slot 0
 };
-
-__device__ __forceinline__ uint64_t stk_get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-    return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * 512u);
-}
-__device__ __forceinline__ void stk_put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t m) {
-    *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * 512u) = m;
-}
 
 // Insert cell i after cell j (iter_good, sym_fold.cpp:210-213).
 __device__ __forceinline__ void place(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
@@ -87,11 +70,77 @@ __device__ __forceinline__ void unplace(uint8_t *lds, const Lane &ln, uint32_t j
     lds[cyc_addr(ln.cl, j)] = lds[cyc_addr(ln.cl, i)];
 }
 
+// ---------------------------------------------------------- stack encodings
+
+// A level's entry: the current child j (ctz of what is left) and the
+// siblings after it.
+struct WideStack {
+    static constexpr uint32_t kSlotBytes = 512;   // u64 x 64 lanes
+    using Entry = uint64_t;
+    static __device__ __forceinline__ uint64_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
+        return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+    }
+    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t e) {
+        *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = e;
+    }
+    static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
+    static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
+    // Siblings after the current child: 0 = none, 1 = next() is the next
+    // child and advance() the new entry.  (Never "recompute".)
+    static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
+    static __device__ __forceinline__ uint64_t advance(uint64_t e) { return e & (e - 1); }
+    // The siblings left as a mask (for splitting); `unknown` set when the
+    // rest must be recomputed above child(e).
+    static __device__ __forceinline__ uint64_t rest(uint64_t e, bool &unknown) {
+        unknown = false;
+        return e & (e - 1);
+    }
+};
+
+// bits 0-5 current child j, bits 6-11 next sibling, bits 12-13 state:
+// 0 = j is the last child, 1 = the next sibling is the last, 2 = more follow
+// the next sibling, 3 = more follow j but are not recorded (recompute).
+struct CompactStack {
+    static constexpr uint32_t kSlotBytes = 128;   // u16 x 64 lanes
+    using Entry = uint32_t;
+    static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
+        return *reinterpret_cast<const uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+    }
+    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
+        *reinterpret_cast<uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = static_cast<uint16_t>(e);
+    }
+    static __device__ __forceinline__ uint32_t encode(uint64_t g) {
+        const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(g));
+        const uint64_t r = g & (g - 1);
+        if (r == 0) return j;
+        const uint32_t j2 = static_cast<uint32_t>(__builtin_ctzll(r));
+        const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
+        return j | (j2 << 6) | (state << 12);
+    }
+    static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & 63u; }
+    // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.
+    static __device__ __forceinline__ uint32_t more(uint32_t e) {
+        const uint32_t s = e >> 12;
+        return s == 0 ? 0u : (s == 3 ? 2u : 1u);
+    }
+    static __device__ __forceinline__ uint32_t advance(uint32_t e) {
+        const uint32_t j2 = (e >> 6) & 63u;
+        return j2 | ((e >> 12) == 2u ? (3u << 12) : 0u);
+    }
+    static __device__ __forceinline__ uint64_t rest(uint32_t e, bool &unknown) {
+        const uint32_t s = e >> 12;
+        unknown = s >= 2u;
+        return s == 1u ? (1ull << ((e >> 6) & 63u)) : 0ull;
+    }
+};
+
 struct Work {
-    uint32_t i;        // next level to expand
+    uint32_t i;        // level of the node to expand next
     uint32_t root;     // item root level: finished when backtracking reaches it
     uint32_t origin;
     uint32_t mult;
+    uint32_t floor;    // 0, or j+1: the next expansion recomputes node i and
+                       // keeps only children above j (not counted as a node)
     uint64_t leaves;   // leaves of the current item so far
     uint32_t inodes;   // nodes, page walks and walk steps of the current item
     uint32_t iwalks;
@@ -99,67 +148,124 @@ struct Work {
     bool have;
 };
 
-// Lane totals over all items (64-bit; item counters are folded in when an
-// item is finished or split).
 struct Totals {
     unsigned long long sum = 0;     // sum of mult * leaves
     unsigned long long nodes = 0, walks = 0, steps = 0, leaves = 0, items = 0;
 };
 
-__device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it, Work &w) {
-    state_to_lds(lds, ln.cl, it->state);
-    const uint64_t mask = it->mask;
-    const uint32_t level = it->level;
-    w.origin = it->origin;
-    w.mult = it->mult;
-    w.root = level;
-    w.leaves = 0;
-    w.inodes = 0;
-    w.iwalks = 0;
-    w.isteps = 0;
-    w.have = true;
-    if (mask == 0) {
-        w.i = level;
-    } else {
-        stk_put(lds, ln, level, mask);
-        place(lds, ln, static_cast<uint32_t>(__builtin_ctzll(mask)), level);
-        w.i = level + 1;
+// get_good (sym_fold.cpp:179-204) for cell i against this lane's cycle.
+__device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
+                                              uint32_t &walks, uint32_t &steps) {
+    uint64_t good = below_mask(i);
+    const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + kPartnerOff)[i];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t anchor = (ew >> (8 * p)) & 0xffu;
+        if (anchor < i && good != 0) {
+            ++walks;
+            uint64_t reach = 1ull << anchor;
+            uint32_t j = lds[cyc_addr(cl, anchor)];
+            while (j != anchor) {
+                ++steps;
+                const uint32_t partner = lds[kPartnerOff + j * 4u + p];
+                j = (partner < i) ? partner : j;
+                reach |= 1ull << j;
+                j = lds[cyc_addr(cl, j)];
+            }
+            good &= reach;
+        }
     }
+    return good;
 }
 
-// One node: expand level w.i, then move to the next node of the subtree (or
-// finish).  Returns true when the item is finished.
-template <bool EV>
+// One node: expand level w.i (or recompute it above w.floor), then move to
+// the next node of the subtree.  Returns true when the item is finished.
+template <typename Stack, bool EV>
 __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t last, Work &w) {
-    uint64_t g = good_mask<EV>(lds, ln.cl, w.i, w.iwalks, w.isteps);
-    ++w.inodes;
-    if (w.i == last) {
-        w.leaves += static_cast<uint32_t>(__builtin_popcountll(g));
-        g = 0;
+    uint32_t nw = 0, ns = 0;
+    uint64_t g = good_mask(lds, ln.cl, w.i, nw, ns);
+    if (w.floor == 0) {
+        ++w.inodes;
+        if (EV) {
+            w.iwalks += nw;
+            w.isteps += ns;
+        }
+        if (w.i == last) {
+            w.leaves += static_cast<uint32_t>(__builtin_popcountll(g));
+            g = 0;
+        }
+    } else {
+        g &= ~below_mask(w.floor);
+        w.floor = 0;
     }
     if (g != 0) {
-        stk_put(lds, ln, w.i, g);
-        place(lds, ln, static_cast<uint32_t>(__builtin_ctzll(g)), w.i);
+        const typename Stack::Entry e = Stack::encode(g);
+        Stack::put(lds, ln, w.i, e);
+        place(lds, ln, Stack::child(e), w.i);
         ++w.i;
         return false;
     }
     for (;;) {
         if (w.i == w.root) return true;
         --w.i;
-        uint64_t m = stk_get(lds, ln, w.i);
-        unplace(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.i);
-        m &= m - 1;
-        if (m != 0) {
-            stk_put(lds, ln, w.i, m);
-            place(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.i);
+        const typename Stack::Entry e = Stack::get(lds, ln, w.i);
+        const uint32_t j = Stack::child(e);
+        unplace(lds, ln, j, w.i);
+        const uint32_t more = Stack::more(e);
+        if (more == 1u) {
+            const typename Stack::Entry e2 = Stack::advance(e);
+            Stack::put(lds, ln, w.i, e2);
+            place(lds, ln, Stack::child(e2), w.i);
             ++w.i;
+            return false;
+        }
+        if (more == 2u) {
+            w.floor = j + 1;
             return false;
         }
     }
 }
 
+// ------------------------------------------------------------ item plumbing
+
+// Item kinds: mask != 0: enter the listed children of node `level`;
+// mask == 0 and map == 0: expand node `level` (counted); mask == 0 and
+// map == f > 0: recompute node `level` and enter its children above f-1.
+__device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it,
+                                                 Work &w) {
+    state_to_lds(lds, ln.cl, it->state);
+    w.origin = it->origin;
+    w.mult = it->mult;
+    w.root = it->level;
+    w.leaves = 0;
+    w.inodes = 0;
+    w.iwalks = 0;
+    w.isteps = 0;
+    w.floor = 0;
+    w.have = true;
+}
+
+template <typename Stack>
+__device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it, Work &w) {
+    load_item_common(lds, ln, it, w);
+    const uint64_t mask = it->mask;
+    const uint32_t level = it->level;
+    if (mask == 0) {
+        w.i = level;
+        w.floor = it->map;
+    } else {
+        // A sibling mask is always every child of the node above some child
+        // (children are entered in ascending order), so an encoding that
+        // records only part of it and recomputes the rest is exact.
+        const typename Stack::Entry e = Stack::encode(mask);
+        Stack::put(lds, ln, level, e);
+        place(lds, ln, Stack::child(e), level);
+        w.i = level + 1;
+    }
+}
+
 // Fold the current item's counters into the lane totals and the per-origin
-// results (used when an item finishes and when it is split).
+// results (when an item finishes and when it is split).
 __device__ __forceinline__ void retire(const SymArgs &a, const Work &w, Totals &t) {
     t.sum += static_cast<unsigned long long>(w.mult) * w.leaves;
     t.nodes += w.inodes;
@@ -184,49 +290,109 @@ __device__ __forceinline__ void flush_wave(const SymArgs &a, uint32_t lane, cons
     }
 }
 
+template <typename Stack>
 __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t wave_bytes = kCycleBytes + levels * 512u;
-    const uint32_t wbase = kEdgeBytes + wave * wave_bytes;
+    const uint32_t wave_bytes = kCycleBytes + levels * Stack::kSlotBytes;
+    const uint32_t wbase = kTableBytes + wave * wave_bytes;
     Lane ln;
     ln.cl = wbase + lane * 4u;
-    ln.sl = wbase + kCycleBytes + lane * 8u;
+    ln.sl = wbase + kCycleBytes + lane * static_cast<uint32_t>(sizeof(typename Stack::Entry) == 8 ? 8 : 2);
     ln.base = base_level;
     return ln;
+}
+
+__device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) reinterpret_cast<uint32_t *>(lds + kPartnerOff)[tid] = a.edge_word[tid];
+    __syncthreads();
+}
+
+// Split everything the lane holds into the spill queue: the node about to be
+// expanded, then for each stack level, deepest first, the siblings not yet
+// entered (a recorded sibling as a mask, unrecorded ones as a recompute piece
+// with a floor).  Called by the whole wave.
+template <typename Stack>
+__device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const Lane &ln, uint32_t lane, Work &w,
+                                          Totals &tot) {
+    bool pend = w.have;
+    uint32_t t = w.i;
+    uint64_t mask = 0;
+    uint32_t floor = w.floor;   // the pending expansion may itself be a recompute
+    bool emit = pend;
+    while (__any(pend)) {
+        uint32_t cnt = 0;
+        const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
+        if (emit) {
+            if (slot < a.spill_cap) {
+                WorkItem *dst = a.spill + slot;
+                lds_to_state(lds, ln.cl, dst->state);
+                dst->mask = mask;
+                dst->origin = w.origin;
+                dst->mult = static_cast<uint16_t>(w.mult);
+                dst->level = static_cast<uint8_t>(t);
+                dst->map = static_cast<uint8_t>(floor);
+            } else {
+                atomicOr(a.acc + ACC_ERROR, 1ull);
+            }
+        }
+        if (cnt != 0 && lane == 0) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
+        emit = false;
+        if (pend) {
+            if (t == w.root) {
+                pend = false;
+            } else {
+                --t;
+                const typename Stack::Entry e = Stack::get(lds, ln, t);
+                const uint32_t j = Stack::child(e);
+                unplace(lds, ln, j, t);
+                bool unknown = false;
+                mask = Stack::rest(e, unknown);
+                floor = 0;
+                if (unknown) {
+                    // Everything above j: recompute node t with floor j.
+                    mask = 0;
+                    floor = j + 1;
+                }
+                emit = mask != 0 || floor != 0;
+            }
+        }
+    }
+    if (w.have) {
+        retire(a, w, tot);
+        w.have = false;
+    }
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- static
-template <bool EV>
+template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const uint32_t tid = threadIdx.x;
-    if (tid < 64) reinterpret_cast<uint32_t *>(lds)[tid] = a.edge_word[tid];
-    __syncthreads();
-    const Lane ln = make_lane(a.levels, a.base_level);
-    const uint32_t idx = blockIdx.x * blockDim.x + tid;
+    stage_tables(a, lds);
+    const Lane ln = make_lane<Stack>(a.levels, a.base_level);
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     Totals tot;
     Work w;
     w.have = false;
-    if (idx < a.num_items) load_item(lds, ln, a.items + idx, w);
+    if (idx < a.num_items) load_item<Stack>(lds, ln, a.items + idx, w);
     while (w.have) {
-        if (dfs_step<EV>(lds, ln, a.last, w)) {
+        if (dfs_step<Stack, EV>(lds, ln, a.last, w)) {
             retire(a, w, tot);
             w.have = false;
         }
     }
-    flush_wave(a, tid & 63u, tot);
+    flush_wave(a, threadIdx.x & 63u, tot);
 }
 
 // ------------------------------------------------------------- persistent
-template <bool EV>
+template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    if (tid < 64) reinterpret_cast<uint32_t *>(lds)[tid] = a.edge_word[tid];
-    __syncthreads();
-    const Lane ln = make_lane(a.levels, a.base_level);
+    stage_tables(a, lds);
+    const uint32_t lane = threadIdx.x & 63u;
+    const Lane ln = make_lane<Stack>(a.levels, a.base_level);
     const unsigned long long below = (1ull << lane) - 1ull;
 
     Totals tot;
@@ -236,8 +402,8 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     uint32_t tail_left = a.tail;
 
     for (;;) {
-        // Idle lanes claim the next input items, one atomic per wave.
         if (!drained) {
+            // Idle lanes claim the next input items, one atomic per wave.
             const unsigned long long idle = __ballot(!w.have);
             if (idle != 0) {
                 const uint32_t want = static_cast<uint32_t>(__popcll(idle));
@@ -248,7 +414,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                 if (got + want >= a.num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < a.num_items) load_item(lds, ln, a.items + idx, w);
+                    if (idx < a.num_items) load_item<Stack>(lds, ln, a.items + idx, w);
                 }
             } else {
                 // Every lane is busy: look at the shared cursor anyway, so that
@@ -259,7 +425,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
         if (!__any(w.have)) break;
 
         for (uint32_t s = 0; s < a.budget && w.have; ++s) {
-            if (dfs_step<EV>(lds, ln, a.last, w)) {
+            if (dfs_step<Stack, EV>(lds, ln, a.last, w)) {
                 retire(a, w, tot);
                 w.have = false;
             }
@@ -268,57 +434,15 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
         if (drained) {
             tail_left = tail_left > a.budget ? tail_left - a.budget : 0;
             if (tail_left == 0) {
-                // Split everything this lane still holds into the spill queue:
-                // first the node about to be expanded, then for each stack level
-                // (deepest first) the siblings not yet entered.
-                bool pend = w.have;
-                uint32_t t = w.i;          // level of the next piece to emit
-                uint64_t mask = 0;         // 0 = expand node t
-                bool emit = pend;
-                while (__any(pend)) {
-                    const unsigned long long em = __ballot(emit);
-                    if (em != 0) {
-                        const uint32_t cnt = static_cast<uint32_t>(__popcll(em));
-                        const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(em));
-                        uint32_t slot0 = 0;
-                        if (lane == first) slot0 = atomicAdd(a.spill_count, cnt);
-                        slot0 = __shfl(slot0, first, 64);
-                        if (emit) {
-                            const uint32_t slot = slot0 + static_cast<uint32_t>(__popcll(em & below));
-                            if (slot < a.spill_cap) {
-                                WorkItem *dst = a.spill + slot;
-                                lds_to_state(lds, ln.cl, dst->state);
-                                dst->mask = mask;
-                                dst->origin = w.origin;
-                                dst->mult = static_cast<uint16_t>(w.mult);
-                                dst->level = static_cast<uint8_t>(t);
-                                dst->map = 0;
-                            } else {
-                                atomicOr(a.acc + ACC_ERROR, 1ull);
-                            }
-                        }
-                        if (lane == first) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
-                    }
-                    // Step up one level: undo that level's current child; the
-                    // remaining siblings become the next piece.
-                    emit = false;
-                    if (pend) {
-                        if (t == w.root) {
-                            pend = false;
-                        } else {
-                            --t;
-                            const uint64_t m = stk_get(lds, ln, t);
-                            unplace(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), t);
-                            mask = m & (m - 1);
-                            emit = mask != 0;
-                        }
+                // Progress guarantee: a split piece restarts the pending
+                // expansion, so every lane first completes one counted node.
+                while (__any(w.have && w.inodes == 0)) {
+                    if (w.have && w.inodes == 0 && dfs_step<Stack, EV>(lds, ln, a.last, w)) {
+                        retire(a, w, tot);
+                        w.have = false;
                     }
                 }
-                if (w.have) {
-                    // Leaves and nodes found so far stay with this lane.
-                    retire(a, w, tot);
-                    w.have = false;
-                }
+                spill_all<Stack>(a, lds, ln, lane, w, tot);
                 break;
             }
         }
@@ -326,32 +450,31 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     flush_wave(a, lane, tot);
 }
 
-// Host-visible launchers (C++ linkage inside the library).
-void launch_sym_static(const SymArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
-                       hipStream_t s) {
-    if (events)
-        hipLaunchKernelGGL(sym_static_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL(sym_static_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+// Host-visible entry points (C++ linkage inside the library).
+namespace {
+template <typename Stack>
+const void *pick(bool persistent, bool events) {
+    if (persistent) return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<Stack, true>)
+                                  : reinterpret_cast<const void *>(&sym_persistent_kernel<Stack, false>);
+    return events ? reinterpret_cast<const void *>(&sym_static_kernel<Stack, true>)
+                  : reinterpret_cast<const void *>(&sym_static_kernel<Stack, false>);
+}
+}  // namespace
+
+// core: 1 = wide u64 stack, otherwise the compact 16-bit stack.
+const void *sym_kernel_ptr(int core, bool persistent, bool events) {
+    return core == 1 ? pick<WideStack>(persistent, events) : pick<CompactStack>(persistent, events);
 }
 
-void launch_sym_persistent(const SymArgs &a, uint32_t grid, uint32_t block, uint32_t lds_bytes, bool events,
-                           hipStream_t s) {
-    if (events)
-        hipLaunchKernelGGL(sym_persistent_kernel<true>, dim3(grid), dim3(block), lds_bytes, s, a);
-    else
-        hipLaunchKernelGGL(sym_persistent_kernel<false>, dim3(grid), dim3(block), lds_bytes, s, a);
+hipError_t launch_sym(const SymArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,
+                      uint32_t lds_bytes, bool events, hipStream_t s) {
+    void *args[] = {const_cast<SymArgs *>(&a)};
+    return hipLaunchKernel(sym_kernel_ptr(core, persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
 }
 
-const void *sym_kernel_ptr(bool persistent, bool events) {
-    if (persistent) return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<true>)
-                                  : reinterpret_cast<const void *>(&sym_persistent_kernel<false>);
-    return events ? reinterpret_cast<const void *>(&sym_static_kernel<true>)
-                  : reinterpret_cast<const void *>(&sym_static_kernel<false>);
-}
-
-uint32_t sym_lds_bytes(uint32_t levels, uint32_t block) {
-    return kEdgeBytes + (block / 64u) * (kCycleBytes + levels * 512u);
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block) {
+    const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
+    return kTableBytes + (block / 64u) * (kCycleBytes + levels * slot);
 }
 
 }  // namespace mf

@@ -18,8 +18,14 @@ BIN = os.path.join(ROOT, "map_folding_amd", "bin")
 SCHEDULES = {
     "static": dict(schedule=0),
     "resplit": dict(schedule=1),
-    "resplit_tight": dict(schedule=1, budget=8, split_levels=1),
+    "resplit_tight": dict(schedule=1, budget=8, tail=8),
+    # the wide u64-mask sym stack (default is the compact one; Lunnon ignores `core`)
+    "static_wide": dict(schedule=0, core=1),
+    "resplit_wide": dict(schedule=1, core=1),
+    "resplit_wide_tight": dict(schedule=1, core=1, budget=4, tail=4),
+    "resplit_tiny": dict(schedule=1, budget=2, tail=2),
 }
+LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight"]
 
 
 def _opts(mf, name, **kw):
@@ -36,7 +42,7 @@ def _say(msg):
 # ------------------------------------------------------------------------------ Lunnon
 
 
-@pytest.mark.parametrize("sched", list(SCHEDULES))
+@pytest.mark.parametrize("sched", LUNNON_SCHEDULES)
 def test_lunnon_5x5_per_state_parity(gpu, oracle, golden, sched):
     _say(f"lunnon 5x5 D15 per-state, schedule={sched}")
     states = gpu.lunnon_frontier([5, 5], 15)
@@ -105,7 +111,8 @@ def _chunk(golden, n, e, start, mx):
     raise KeyError((n, e, start, mx))
 
 
-@pytest.mark.parametrize("sched", ["resplit", "resplit_tight"])
+@pytest.mark.parametrize("sched", ["resplit", "resplit_tight", "resplit_tiny", "resplit_wide",
+                                   "resplit_wide_tight"])
 def test_sym_8x8_first_300_events(gpu, golden, sched):
     _say(f"sym 8x8 d36 [0,300) schedule={sched}")
     g = _chunk(golden, 8, 36, 0, 300)
@@ -135,7 +142,7 @@ def test_sym_per_case_parity_vs_oracle(gpu, oracle):
         cyc, mul = _small_cases(gpu, n, e, start, k, 400000)
         assert cyc.shape[0] >= 20
         cnt, nodes, tot, st = oracle.sym_solve(n, e, cyc, mul)
-        for sched in ("static", "resplit_tight"):
+        for sched in ("static", "resplit_tight", "resplit_tiny", "static_wide", "resplit_wide_tight"):
             res, pc, pn = gpu.sym_solve(n, e, cyc, mul, _opts(gpu, sched), per_item=True)
             assert np.array_equal(pc, cnt), sched
             assert np.array_equal(pn, nodes), sched
