@@ -238,7 +238,8 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
                 HIP_TRY(hipEventElapsedTime(&one, g.la, g.lb));
                 launch_ms += one;
                 ++rounds;
-                if (g.host_acc[ACC_ERROR] & 2ull) return fail(MF_ERR_INTERNAL, "a state walk did not close");
+                if (g.host_acc[ACC_ERROR] & 6ull) return fail(MF_ERR_INTERNAL, "device consistency check failed (%llu)",
+                                                               g.host_acc[ACC_ERROR]);
                 if (g.host_acc[ACC_ERROR] != 0)
                     return fail(MF_ERR_CAPACITY, "spill queue overflow (capacity %llu items)",
                                 static_cast<unsigned long long>(cap));
@@ -255,7 +256,8 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
     if (per_nodes) HIP_TRY(hipMemcpyAsync(per_nodes, dnodes, origins * 8, hipMemcpyDeviceToHost, g.stream));
     if (per_map) HIP_TRY(hipMemcpyAsync(per_map, dmap, static_cast<size_t>(nmaps) * 8, hipMemcpyDeviceToHost, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    if (g.host_acc[ACC_ERROR] & 2ull) return fail(MF_ERR_INTERNAL, "a state walk did not close");
+    if (g.host_acc[ACC_ERROR] & 6ull)
+        return fail(MF_ERR_INTERNAL, "device consistency check failed (%llu)", g.host_acc[ACC_ERROR]);
     if (g.host_acc[ACC_ERROR] != 0) return fail(MF_ERR_CAPACITY, "work queue overflow");
     fill_result(out, g.host_acc);
     out->rounds = rounds;

@@ -46,10 +46,48 @@ namespace mf {
 namespace {
 
 constexpr uint32_t kPartnerOff = 0;      // 64 words: byte p = partner on page p (n2 = none)
-constexpr uint32_t kTableBytes = 256;
+constexpr uint32_t kPageOff = 256;       // the same, page-major: byte p*64 + c
+constexpr uint32_t kTableBytes = 512;
 constexpr uint32_t kCycleBytes = 4096;   // 16 words x 64 lanes
 
+// Cycle layout.  Byte-interleaved (default): cell c of lane L at byte
+// c*64 + L -- one VALU op per address; the four lanes of a quad share
+// dwords, so lanes of a quad looking up different cells can conflict.
+// Word-interleaved (MF_SYM_BYTE_LAYOUT=0): dword w of lane L holds cells
+// 4w..4w+3 at word w*64 + L, so lanes never share a bank, at four VALU ops
+// per address.  Measured on the 8x8 depth-36 chunk: byte 1.63 s, word 1.82 s.
+#ifndef MF_SYM_BYTE_LAYOUT
+#define MF_SYM_BYTE_LAYOUT 1
+#endif
+#if MF_SYM_BYTE_LAYOUT
+__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return lane_base + (cell << 6); }
+__device__ __forceinline__ uint32_t cyc_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
+__device__ __forceinline__ void cyc_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = s4[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+    }
+}
+__device__ __forceinline__ void cyc_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+        d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+#else
 __device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_addr(lane_base, cell); }
+__device__ __forceinline__ uint32_t cyc_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane * 4u; }
+__device__ __forceinline__ void cyc_load(uint8_t *lds, uint32_t lb, const uint8_t *src) { state_to_lds(lds, lb, src); }
+__device__ __forceinline__ void cyc_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) { lds_to_state(lds, lb, dst); }
+#endif
 
 __device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ? ~0ull : ((1ull << i) - 1ull); }
 
@@ -167,7 +205,10 @@ __device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, u
             uint32_t j = lds[cyc_addr(cl, anchor)];
             while (j != anchor) {
                 ++steps;
-                const uint32_t partner = lds[kPartnerOff + j * 4u + p];
+                // Absolute LDS address (the kernel's only LDS is the dynamic
+                // block at offset 0): the page offset folds into the ds offset.
+                const uint32_t partner =
+                    *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(kPageOff + 64u * p + j);
                 j = (partner < i) ? partner : j;
                 reach |= 1ull << j;
                 j = lds[cyc_addr(cl, j)];
@@ -233,7 +274,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
 // map == f > 0: recompute node `level` and enter its children above f-1.
 __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it,
                                                  Work &w) {
-    state_to_lds(lds, ln.cl, it->state);
+    cyc_load(lds, ln.cl, it->state);
     w.origin = it->origin;
     w.mult = it->mult;
     w.root = it->level;
@@ -296,7 +337,7 @@ __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) 
     const uint32_t wave_bytes = kCycleBytes + levels * Stack::kSlotBytes;
     const uint32_t wbase = kTableBytes + wave * wave_bytes;
     Lane ln;
-    ln.cl = wbase + lane * 4u;
+    ln.cl = cyc_lane_base(wbase, lane);
     ln.sl = wbase + kCycleBytes + lane * static_cast<uint32_t>(sizeof(typename Stack::Entry) == 8 ? 8 : 2);
     ln.base = base_level;
     return ln;
@@ -304,7 +345,15 @@ __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) 
 
 __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
     const uint32_t tid = threadIdx.x;
-    if (tid < 64) reinterpret_cast<uint32_t *>(lds + kPartnerOff)[tid] = a.edge_word[tid];
+    // The walk addresses the page table by absolute LDS offset.
+    const auto *local = (const __attribute__((address_space(3))) uint8_t *)lds;
+    if (tid == 0 && static_cast<uint32_t>(reinterpret_cast<size_t>(local)) != 0u) atomicOr(a.acc + ACC_ERROR, 4ull);
+    if (tid < 64) {
+        const uint32_t w = a.edge_word[tid];
+        reinterpret_cast<uint32_t *>(lds + kPartnerOff)[tid] = w;
+#pragma unroll
+        for (uint32_t p = 0; p < 4; ++p) lds[kPageOff + 64u * p + tid] = static_cast<uint8_t>(w >> (8u * p));
+    }
     __syncthreads();
 }
 
@@ -326,7 +375,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         if (emit) {
             if (slot < a.spill_cap) {
                 WorkItem *dst = a.spill + slot;
-                lds_to_state(lds, ln.cl, dst->state);
+                cyc_store(lds, ln.cl, dst->state);
                 dst->mask = mask;
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.mult);
