@@ -10,10 +10,12 @@ for the default chunk) on every timed step.
 
 Prints ONE JSON line.  `value` is DFS nodes/s (node = one get_good call) of
 the batched persistent re-split schedule; the plain static-frontier kernel
-(one lane per case, one launch) is timed on the same chunk beside it.
+(one lane per case, one launch) is timed beside it on the chunk's first
+--plain-cases cases (it is bounded by its longest case: minutes on the whole
+chunk).  Other configurations: --workload sym7 | lunnon6 | rect64.
 
-    python bench.py [--steps K] [--warmup W] [--start S] [--cases C]
-                    [--no-plain] [--no-cpu] [--cpu-cases M]
+    python bench.py [--steps K] [--warmup W] [--workload W] [--start S] [--cases C]
+                    [--no-plain] [--no-cpu] [--cpu-cases M] [--plain-cases P]
 """
 import argparse
 import json
@@ -25,6 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 GOLDEN_CHUNK = (120000000, 50000, 26687875796992)   # published/reference chunk total (x n^2)
+# Symmetry-DFS workloads: (n, expanded level, default start, cases, golden answer of that window).
+SYM_WORKLOADS = {
+    "sym8": (8, 36, 120000000, 50000, 26687875796992),
+    "sym7": (7, 24, 200000, 20000, 4968634597136),
+}
 
 
 def log(msg):
@@ -35,7 +42,7 @@ def _traffic(args):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this same command (scaled by
     the copy-kernel calibration); None for another workload."""
-    if (args.start, args.cases) != GOLDEN_CHUNK[:2]:
+    if args.workload != "sym8" or (args.start, args.cases) != GOLDEN_CHUNK[:2]:
         return None
     path = os.path.join(ROOT, "profiles", "sym_persistent_8x8_d36_chunk_traffic.json")
     if not os.path.exists(path):
@@ -44,17 +51,136 @@ def _traffic(args):
         return json.load(f)["hbm_bytes_per_launch"]
 
 
+def rect_shapes(seed, k=64):
+    """The batched-instances draw: p uniform in {2,3,4}, then q uniform among
+    the q with 12 <= p*q <= 30."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        p = int(rng.integers(2, 5))
+        qs = [q for q in range(1, 31) if 12 <= p * q <= 30]
+        out.append((p, int(rng.choice(qs))))
+    return out
+
+
+def rect_depth(p, q):
+    """Frontier depth per map: deep enough that every map splits into many
+    states, shallow enough that the host collector stays cheap."""
+    return max(2, min(p * q - 6, (2 * p * q) // 3))
+
+
+def run_lunnon(args, mf, rl):
+    """lunnon6: 6x6 map, frontier at depth 18 (112,024 states), resumed on the
+    GPU; rect64: 64 seeded rectangles, each with its own frontier, all in one
+    device queue.  Totals checked against the golden table."""
+    import numpy as np
+    with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+        golden = json.load(f)
+    mf.init(0)
+    cus = mf.device_cus()
+    peak = rl.int32_peak_ops(cus or 256) / 1e12
+    batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
+                              tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
+    events = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core)
+    t0 = time.time()
+    if args.workload == "lunnon6":
+        dims, depth = [6, 6], 18
+        states = mf.lunnon_frontier(dims, depth)
+        front = mf.LunnonFrontier(dims, depth, states)
+        solve = front.solve
+        want = golden["square_totals"]["6"]
+        label = f"6x6 Lunnon DFS, partition depth 18 ({states.shape[0]} states)"
+        items = states.shape[0]
+    else:
+        shapes = rect_shapes(args.seed)
+        depths = [rect_depth(p, q) for p, q in shapes]
+        want_maps = [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
+        want = sum(want_maps)
+        batch = mf.LunnonBatchFrontier(shapes, depths)
+
+        def solve(opt):
+            totals, res = batch.solve_maps(opt)
+            if totals != want_maps:
+                raise SystemExit("a map total disagreed with the golden table")
+            return res, None, None
+        label = f"64 rectangles (seed {args.seed}), per-map frontier depths {min(depths)}..{max(depths)}"
+        items = len(shapes)
+    gen_s = time.time() - t0
+    res, _, _ = solve(events)
+    if res.count != want:
+        raise SystemExit(f"total {res.count} != golden {want}")
+    log(f"{label}: counting run total={res.count} inserts={res.nodes} gapcomps={res.walks} steps={res.steps}")
+    for _ in range(args.warmup):
+        solve(batched)
+    wall, kern = [], []
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        r, _, _ = solve(batched)
+        wall.append(time.perf_counter() - t1)
+        kern.append(r.kernel_ms / 1e3)
+        if r.count != want:
+            raise SystemExit("a timed step disagreed")
+        log(f"step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}")
+    ms = 1e3 * sum(wall) / len(wall)
+    ops = rl.lunnon_ops(res.walks, res.steps, res.cands, res.nodes, res.leaves)
+    achieved = ops / (sum(kern) / len(kern)) / 1e12
+    plain = None
+    if not args.no_plain and args.workload == "lunnon6":
+        t1 = time.perf_counter()
+        rp, _, _ = solve(mf.SolveOptions(schedule=mf.MF_SCHED_STATIC, count_events=False, core=args.core))
+        pw = time.perf_counter() - t1
+        if rp.count != want:
+            raise SystemExit("static schedule disagreed")
+        plain = {"nodes_per_s": res.nodes / pw, "ms": pw * 1e3, "speedup_batched_vs_plain": pw * 1e3 / ms}
+        log(f"plain static: {pw*1e3:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        t1 = time.perf_counter()
+        if args.workload == "lunnon6":
+            m = min(args.cpu_cases, states.shape[0])
+            _, _, _, cst = oracle.lunnon_resume([6, 6], 18, states[:m])
+            sample = f"first {m} frontier states, oracle C restatement of mf.c / solve_from_state, 1 thread"
+        else:
+            small = [(p, q) for p, q in shapes if p * q <= 20][:8]
+            cst = {"inserts": 0}
+            for p, q in small:
+                cst["inserts"] += oracle.lunnon_count([p, q])[1]["inserts"]
+            sample = f"full counts of {small} (oracle C restatement of mf.c), 1 thread"
+        cs = time.perf_counter() - t1
+        cpu = {"value": cst["inserts"] / cs, "unit": "inserts/s", "cores": 1, "kind": "port",
+               "sample": f"{sample}, {cs:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3e} inserts/s")
+    return {
+        "metric": "DFS nodes/s & cases/s on 1 MI355X, batched+learned vs plain, bit-exact",
+        "value": res.nodes / (ms / 1e3), "unit": "inserts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic: deterministic Lunnon frontiers",
+        "config": {"workload": label, "items": items, "total": str(res.count), "bit_exact_vs_golden": True},
+        "items_per_s": items / (ms / 1e3), "nodes": res.nodes, "gapcomps": res.walks, "walk_steps": res.steps,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s", "frac": achieved / peak,
+                     "traffic": None, "ops_per_step": ops,
+                     "ops_model": "10*gapcomps + 8*walk_steps + 6*candidates + 8*inserts + 2*completions"},
+        "plain": plain, "cpu_baseline": cpu, "frontier_gen_s": gen_s,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--start", type=int, default=GOLDEN_CHUNK[0])
-    ap.add_argument("--cases", type=int, default=GOLDEN_CHUNK[1])
+    ap.add_argument("--workload", default="sym8", choices=["sym8", "sym7", "lunnon6", "rect64"],
+                    help="sym8 (default): 8x8 depth-36 50k-case chunk; sym7: 7x7 depth-24 20k cases; "
+                         "lunnon6: 6x6 Lunnon frontier at depth 18; rect64: 64 seeded rectangles in one queue")
+    ap.add_argument("--start", type=int, default=None)
+    ap.add_argument("--cases", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain static-frontier kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cases", type=int, default=4000)
     ap.add_argument("--plain-cases", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=0, help="rect64: seed of the shape draw")
     ap.add_argument("--budget", type=int, default=0)
     ap.add_argument("--tail", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=0)
@@ -73,7 +199,14 @@ def main():
     import map_folding_amd as mf
     from map_folding_amd import roofline as rl
 
-    n, level = 8, 36
+    if args.workload in ("lunnon6", "rect64"):
+        print(json.dumps(run_lunnon(args, mf, rl)), flush=True)
+        return
+    n, level, start0, cases0, golden0 = SYM_WORKLOADS[args.workload]
+    if args.start is None:
+        args.start = start0
+    if args.cases is None:
+        args.cases = cases0
     mf.init(0)
     cus = mf.device_cus()
     t0 = time.time()
@@ -88,7 +221,7 @@ def main():
     res, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core))
     nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
     total = res.count * n * n
-    golden = GOLDEN_CHUNK[2] if (args.start, args.cases) == GOLDEN_CHUNK[:2] else None
+    golden = golden0 if (args.start, args.cases) == (start0, cases0) else None
     if golden is not None and total != golden:
         raise SystemExit(f"chunk total {total} != golden {golden}")
     log(f"counting run: total={total} nodes={nodes} walks={walks} steps={steps} rounds={res.rounds} "
@@ -170,7 +303,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic: deterministic sym_fold case stream (no dataset)",
-        "config": {"workload": f"8x8 sym_fold MF_EXPANDED_N=36 chunk [{args.start},{args.start + cyc.shape[0]})",
+        "config": {"workload": f"{n}x{n} sym_fold MF_EXPANDED_N={level} chunk [{args.start},{args.start + cyc.shape[0]})",
                    "cases": int(cyc.shape[0]), "schedule": "persistent re-split", "chunk_total": str(total),
                    "bit_exact_vs_golden": golden is not None},
         "cases_per_s": case_rate,

@@ -164,9 +164,15 @@ int mf_sym_frontier_create(int32_t n, int32_t level, const uint8_t *cycles, cons
                            int64_t num, mf_frontier **out);
 int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth,
                               const uint8_t *states_b, int64_t num, mf_frontier **out);
+/* Batched uneven instances as a resident frontier (see mf_lunnon_batch). */
+int mf_lunnon_batch_frontier_create(const int32_t *dims2, const int32_t *depths, int32_t nmaps,
+                                    mf_frontier **out);
 /* Solve a resident frontier; per_count/per_nodes are host arrays (nullable). */
 int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt,
                       uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
+/* Solve a resident Lunnon frontier and return each map's exact total. */
+int mf_frontier_solve_maps(mf_frontier *f, const mf_solve_options *opt,
+                           uint64_t *per_map_count, mf_result *out);
 void mf_frontier_destroy(mf_frontier *f);
 
 /* ---- sym_kernel.c-compatible batch ABI (mfah wire format) --------------- */

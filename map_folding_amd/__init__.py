@@ -244,6 +244,29 @@ class SymFrontier(_Frontier):
         super().__init__(h, cycles.shape[0])
 
 
+class LunnonBatchFrontier(_Frontier):
+    """Many 2-D maps, each with its own frontier depth, resident as one queue."""
+
+    def __init__(self, shapes, depths):
+        shapes = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 2))
+        depths = np.ascontiguousarray(np.asarray(depths, dtype=np.int32))
+        h = C.c_void_p()
+        check(lib().mf_lunnon_batch_frontier_create(_p(shapes, C.c_int32), _p(depths, C.c_int32),
+                                                    shapes.shape[0], C.byref(h)), "mf_lunnon_batch_frontier_create")
+        super().__init__(h, 0)
+        self.nmaps = shapes.shape[0]
+
+    def solve_maps(self, options: SolveOptions | None = None):
+        """(per-map exact totals, MFResult)."""
+        out = np.zeros(self.nmaps, dtype=np.uint64)
+        res = MFResult()
+        o, keep = _opts(options)
+        check(lib().mf_frontier_solve_maps(self._h, C.byref(o), _p(out, C.c_uint64), C.byref(res)),
+              "mf_frontier_solve_maps")
+        del keep
+        return [int(x) for x in out], res
+
+
 class LunnonFrontier(_Frontier):
     def __init__(self, dims, depth: int, states: np.ndarray):
         d, dp = _dims(dims)

@@ -93,6 +93,13 @@ def lib() -> C.CDLL:
         "mf_sym_frontier_create": (C.c_int, [i32, i32, P(u8), P(u64), i64, P(C.c_void_p)]),
         "mf_lunnon_frontier_create": (C.c_int, [P(i32), i32, i32, P(u8), i64, P(C.c_void_p)]),
         "mf_frontier_solve": (C.c_int, [C.c_void_p, P(MFSolveOptions), P(u64), P(u64), P(MFResult)]),
+        "mf_lunnon_batch_frontier_create": (C.c_int, [P(i32), P(i32), i32, P(C.c_void_p)]),
+        "mf_frontier_solve_maps": (C.c_int, [C.c_void_p, P(MFSolveOptions), P(u64), P(MFResult)]),
+        "mf_wasm_input_ptr": (P(u8), []),
+        "mf_wasm_output_ptr": (P(u32), []),
+        "mf_wasm_case_stride": (u32, []),
+        "mf_wasm_max_cases": (u32, []),
+        "mf_wasm_solve_cases": (u32, [u32, u32, u32, u32]),
         "mf_frontier_destroy": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():

@@ -23,10 +23,10 @@ hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid,
                       bool events, hipStream_t);
 uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block);
 const void *sym_kernel_ptr(int core, bool persistent, bool events);
-hipError_t launch_lunnon(const LunnonArgs &, bool persistent, uint32_t grid, uint32_t block, uint32_t lds,
-                         bool events, hipStream_t);
-uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block);
-const void *lunnon_kernel_ptr(bool persistent, bool events);
+hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t grid, uint32_t block,
+                         uint32_t lds, bool events, hipStream_t);
+uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block);
+const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 }  // namespace mf
 
 using namespace mf;
@@ -374,15 +374,16 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         HIP_TRY(hipMemcpyAsync(g.tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream));
         a.tables = static_cast<const uint8_t *>(g.tables.ptr);
         a.per_map = nullptr;
-        const uint32_t lds = lunnon_lds_bytes(a.tables_bytes, levels, t.block);
+        const uint32_t lds = lunnon_lds_bytes(t.core, a.tables_bytes, levels, t.block);
         if (per_map) {
             if ((rc = g.per_map.ensure(f->maps.size() * 8)) != MF_OK) return rc;
             a.per_map = static_cast<unsigned long long *>(g.per_map.ptr);
         }
         auto launch = [&](const LunnonArgs &args, bool persistent, uint32_t grid) {
-            return launch_lunnon(args, persistent, grid, t.block, lds, t.events, g.stream);
+            return launch_lunnon(args, t.core, persistent, grid, t.block, lds, t.events, g.stream);
         };
-        rc = run_solve(a, lunnon_kernel_ptr(false, t.events), lunnon_kernel_ptr(true, t.events), lds, t, dev_items,
+        rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events), lunnon_kernel_ptr(t.core, true, t.events), lds,
+                       t, dev_items,
                        f->num, f->num, per_count, per_nodes, per_map, static_cast<uint32_t>(f->maps.size()), out,
                        launch);
     }
@@ -561,31 +562,59 @@ int mf_gpu_folding(const int32_t *dims, int32_t ndim, int32_t depth, const mf_so
     return mf_lunnon_solve(dims, ndim, depth, states.data(), total, opt, nullptr, nullptr, out);
 }
 
-int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps, const mf_solve_options *opt,
-                    uint64_t *per_map_count, mf_result *out) {
+int mf_lunnon_batch_frontier_create(const int32_t *dims2, const int32_t *depths, int32_t nmaps, mf_frontier **out) {
     int rc = require_init();
     if (rc != MF_OK) return rc;
     if (!dims2 || !depths || !out || nmaps < 1 || nmaps > kMaxMaps) return fail(MF_ERR_ARG, "bad batch arguments");
-    mf_frontier f;
-    f.kind = 2;
+    auto *f = new mf_frontier();
+    f->kind = 2;
     for (int k = 0; k < nmaps; ++k) {
         LunnonProblem pr;
         std::string err;
-        if (!lunnon_problem(dims2 + 2 * k, 2, pr, err)) return fail(MF_ERR_ARG, "map %d: %s", k, err.c_str());
+        if (!lunnon_problem(dims2 + 2 * k, 2, pr, err)) {
+            delete f;
+            return fail(MF_ERR_ARG, "map %d: %s", k, err.c_str());
+        }
         const int d = depths[k];
-        if (d < 1 || d >= pr.n) return fail(MF_ERR_DEPTH, "map %d: depth must be between 1 and %d", k, pr.n - 1);
+        if (d < 1 || d >= pr.n) {
+            delete f;
+            return fail(MF_ERR_DEPTH, "map %d: depth must be between 1 and %d", k, pr.n - 1);
+        }
         const int64_t cnt = lunnon_frontier(pr, d, nullptr, 0);
         std::vector<uint8_t> st(static_cast<size_t>(cnt) * (d + 1));
         lunnon_frontier(pr, d, st.data(), cnt);
         for (int64_t s = 0; s < cnt; ++s)
-            f.host_items.push_back(lunnon_item(st.data() + s * (d + 1), d, pr.n,
-                                               static_cast<uint32_t>(f.host_items.size()), k));
-        f.maps.push_back(pr);
-        f.depths.push_back(d);
+            f->host_items.push_back(lunnon_item(st.data() + s * (d + 1), d, pr.n,
+                                                static_cast<uint32_t>(f->host_items.size()), k));
+        f->maps.push_back(pr);
+        f->depths.push_back(d);
     }
-    if ((rc = upload_items(&f)) != MF_OK) return rc;
-    rc = solve_frontier(&f, opt, nullptr, nullptr, per_map_count, out);
-    f.items.release();
+    if (f->host_items.size() > 0xffffffffull) {
+        delete f;
+        return fail(MF_ERR_CAPACITY, "too many states for one frontier");
+    }
+    if ((rc = upload_items(f)) != MF_OK) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return MF_OK;
+}
+
+int mf_frontier_solve_maps(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_map_count, mf_result *out) {
+    if (!f || !out || !per_map_count) return fail(MF_ERR_ARG, "frontier/out/per_map_count is NULL");
+    if (f->kind != 2) return fail(MF_ERR_ARG, "per-map totals exist for Lunnon frontiers only");
+    return solve_frontier(f, opt, nullptr, nullptr, per_map_count, out);
+}
+
+int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps, const mf_solve_options *opt,
+                    uint64_t *per_map_count, mf_result *out) {
+    if (!out || !per_map_count) return fail(MF_ERR_ARG, "out/per_map_count is NULL");
+    mf_frontier *f = nullptr;
+    int rc = mf_lunnon_batch_frontier_create(dims2, depths, nmaps, &f);
+    if (rc != MF_OK) return rc;
+    rc = mf_frontier_solve_maps(f, opt, per_map_count, out);
+    mf_frontier_destroy(f);
     return rc;
 }
 

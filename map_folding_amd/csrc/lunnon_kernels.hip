@@ -3,52 +3,51 @@
 // Restates the resume loop of mf.c:78-143 / solve_state_device
 // (gpu_folding_v2.cu:381-433) per lane:
 //   * state = the "below" links b[0..l-1] of the placed leaves (a circular
-//     list rooted at leaf 0); a[] is never needed because the leaf a new leaf
-//     was linked after is the stack's current child;
+//     list rooted at leaf 0), one byte row per lane in LDS; a[] is never
+//     needed because the leaf a new leaf was linked below is the stack's
+//     current child;
 //   * the gap computation of leaf l (mf.c:88-124) walks each dimension's
 //     chain m -> d[i][l][b[m]] and keeps the positions reached by every
-//     dimension that constrains l -- here as u64 masks instead of a gap stack
-//     and a per-leaf tally (positions are < l <= 64);
+//     dimension that constrains l -- as u64 masks instead of a gap stack and
+//     a per-leaf tally (positions are < l <= 64);
 //   * every legal position is one insert (node).  An insert at position 0 of
 //     a leaf l >= 2 breaks the normal-form filter b[0] == 1 (mf.c:83) and is
 //     pruned at once, so it is counted but never entered; an insert of leaf n
 //     is a completion worth +n (mf.c:84-85, process() :9-11);
-//   * b rows and the per-level mask stack are lane-interleaved in LDS; the
-//     d tables of every map in the launch sit in LDS as bytes.
+//   * the stack uses the encodings of mf_dfs.hpp (compact by default: a
+//     level whose unrecorded positions are needed recomputes its gaps,
+//     uncounted, above the current one);
+//   * the d tables of every map in the launch sit in LDS as bytes.
 // Child order differs from the reference's gap-stack order; subtree counts,
 // inserts and completions do not depend on it.
 #include <hip/hip_runtime.h>
 
 #include "mf_device.hpp"
+#include "mf_dfs.hpp"
 #include "mf_wave.hpp"
 
 namespace mf {
 namespace {
 
-constexpr uint32_t kRowBytes = 4096;   // 16 words x 64 lanes: b[0..63]
 constexpr uint32_t kMapBytes = kMaxMaps * sizeof(LunnonMap);
 
-struct LLane {
-    uint32_t bl;      // LDS byte address of this lane's b row
-    uint32_t sl;      // LDS byte address of this lane's stack slot 0
-};
-
 struct LWork {
-    uint32_t l;       // next leaf to insert
+    uint32_t l;       // next leaf to insert (the node being expanded)
     uint32_t root;
     uint32_t n;
     uint32_t ndim;
     uint32_t tab;     // LDS byte address of the map's d table
-    uint32_t base;    // leaf of stack slot 0
     uint32_t origin;
     uint32_t map;
+    uint32_t floor;   // 0, or j+1: recompute the gaps of leaf l, keep those above j
     uint64_t comps;   // completions of the current item
-    uint32_t inodes;  // inserts, gap computations, walk steps of the current item
+    uint32_t inodes;  // inserts, gap computations, walk steps, candidates of the item
     uint32_t iwalks;
     uint32_t isteps;
     uint32_t icands;
     bool have;
-    bool bad;         // a walk exceeded n steps: the state was not a folding state
+    bool progressed;  // a counted gap computation of this item has completed
+    bool bad;         // a walk exceeded l steps: the state was not a folding state
 };
 
 struct LTotals {
@@ -56,31 +55,24 @@ struct LTotals {
     unsigned long long nodes = 0, walks = 0, steps = 0, cands = 0, comps = 0, items = 0;
 };
 
-__device__ __forceinline__ uint64_t lstk_get(const uint8_t *lds, const LLane &ln, const LWork &w, uint32_t l) {
-    return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (l - w.base) * 512u);
-}
-__device__ __forceinline__ void lstk_put(uint8_t *lds, const LLane &ln, const LWork &w, uint32_t l, uint64_t m) {
-    *reinterpret_cast<uint64_t *>(lds + ln.sl + (l - w.base) * 512u) = m;
-}
-
 // Link leaf l directly below leaf m (mf.c:137-140 without a[]).
-__device__ __forceinline__ void link(uint8_t *lds, const LLane &ln, uint32_t m, uint32_t l) {
-    lds[row_addr(ln.bl, l)] = lds[row_addr(ln.bl, m)];
-    lds[row_addr(ln.bl, m)] = static_cast<uint8_t>(l);
+__device__ __forceinline__ void link(uint8_t *lds, const Lane &ln, uint32_t m, uint32_t l) {
+    lds[row_at(ln.cl, l)] = lds[row_at(ln.cl, m)];
+    lds[row_at(ln.cl, m)] = static_cast<uint8_t>(l);
 }
 // Unlink leaf l that sits below leaf m (mf.c:131-132).
-__device__ __forceinline__ void unlink(uint8_t *lds, const LLane &ln, uint32_t m, uint32_t l) {
-    lds[row_addr(ln.bl, m)] = lds[row_addr(ln.bl, l)];
+__device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m, uint32_t l) {
+    lds[row_at(ln.cl, m)] = lds[row_at(ln.cl, l)];
 }
 
 // Legal positions for leaf l: mask of m < l reached by the walk of every
-// dimension with d[i][l][l] != l (all m < l if none constrains l).
-// A walk visits each position at most once, so it ends within l steps; the
-// guard only trips on a state that no folding DFS can produce.
-template <bool EV>
-__device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const LLane &ln, LWork &w) {
+// dimension with d[i][l][l] != l (all m < l if none constrains l).  A walk
+// visits each position at most once, so it ends within l steps; the guard only
+// trips on a state that no folding DFS can produce.
+__device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
+                                             uint32_t &steps, uint32_t &cands) {
     const uint32_t l = w.l, s = w.n + 1;
-    const uint64_t all = (l >= 64u) ? ~0ull : ((1ull << l) - 1ull);
+    const uint64_t all = below_mask(l);
     uint64_t meet = all, reach_any = 0;
     bool constrained = false;
     for (uint32_t i = 0; i < w.ndim; ++i) {
@@ -91,9 +83,9 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         uint64_t reach = 0;
         uint32_t guard = 0;
         do {
-            if (EV) ++w.isteps;
+            ++steps;
             reach |= 1ull << m;
-            m = lds[row + lds[row_addr(ln.bl, m)]];
+            m = lds[row + lds[row_at(ln.cl, m)]];
             if (++guard > l) {
                 w.bad = true;
                 break;
@@ -102,21 +94,71 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         meet &= reach;
         reach_any |= reach;
     }
-    if (EV) {
-        ++w.iwalks;
-        w.icands += constrained ? static_cast<uint32_t>(__popcll(reach_any)) : l;
-    }
+    cands += constrained ? static_cast<uint32_t>(__popcll(reach_any)) : l;
     return constrained ? meet : all;
 }
 
-__device__ __forceinline__ void lload_item(uint8_t *lds, const LLane &ln, const WorkItem *__restrict__ it,
+template <typename Stack, bool EV>
+__device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w) {
+    uint32_t ns = 0, nc = 0;
+    uint64_t g = gap_mask(lds, ln, w, ns, nc);
+    if (w.bad) return true;
+    if (w.floor == 0) {
+        w.progressed = true;
+        w.inodes += static_cast<uint32_t>(__popcll(g));
+        if (EV) {
+            ++w.iwalks;
+            w.isteps += ns;
+            w.icands += nc;
+        }
+        g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
+        if (w.l == w.n) {
+            w.comps += static_cast<uint32_t>(__popcll(g));
+            g = 0;
+        }
+    } else {
+        g &= ~below_mask(w.floor);
+        w.floor = 0;
+    }
+    if (g != 0) {
+        const typename Stack::Entry e = Stack::encode(g);
+        Stack::put(lds, ln, w.l, e);
+        link(lds, ln, Stack::child(e), w.l);
+        ++w.l;
+        return false;
+    }
+    for (;;) {
+        if (w.l == w.root) return true;
+        --w.l;
+        const typename Stack::Entry e = Stack::get(lds, ln, w.l);
+        const uint32_t j = Stack::child(e);
+        unlink(lds, ln, j, w.l);
+        const uint32_t more = Stack::more(e);
+        if (more == 1u) {
+            const typename Stack::Entry e2 = Stack::advance(e);
+            Stack::put(lds, ln, w.l, e2);
+            link(lds, ln, Stack::child(e2), w.l);
+            ++w.l;
+            return false;
+        }
+        if (more == 2u) {
+            w.floor = j + 1;
+            return false;
+        }
+    }
+}
+
+// Item kinds as for the symmetry DFS; the floor of a recompute piece travels
+// in the upper byte of `mult` (the low byte is n).
+template <typename Stack>
+__device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkItem *__restrict__ it,
                                            const LunnonMap *maps, LWork &w) {
-    state_to_lds(lds, ln.bl, it->state);
+    row_load(lds, ln.cl, it->state);
     const LunnonMap mp = maps[it->map];
     w.n = mp.n;
     w.ndim = mp.ndim;
     w.tab = mp.table_off;
-    w.base = mp.base;
+    ln.base = mp.base;
     w.map = it->map;
     w.origin = it->origin;
     const uint32_t level = it->level;
@@ -127,45 +169,18 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, const LLane &ln, const 
     w.iwalks = 0;
     w.isteps = 0;
     w.icands = 0;
+    w.floor = 0;
     w.have = true;
+    w.progressed = false;
     w.bad = false;
     if (mask == 0) {
         w.l = level;
+        w.floor = it->mult >> 8;
     } else {
-        lstk_put(lds, ln, w, level, mask);
-        link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(mask)), level);
+        const typename Stack::Entry e = Stack::encode(mask);
+        Stack::put(lds, ln, level, e);
+        link(lds, ln, Stack::child(e), level);
         w.l = level + 1;
-    }
-}
-
-template <bool EV>
-__device__ __forceinline__ bool ldfs_step(uint8_t *lds, const LLane &ln, LWork &w) {
-    uint64_t g = gap_mask<EV>(lds, ln, w);
-    if (w.bad) return true;
-    w.inodes += static_cast<uint32_t>(__popcll(g));
-    g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
-    if (w.l == w.n) {
-        w.comps += static_cast<uint32_t>(__popcll(g));
-        g = 0;
-    }
-    if (g != 0) {
-        lstk_put(lds, ln, w, w.l, g);
-        link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(g)), w.l);
-        ++w.l;
-        return false;
-    }
-    for (;;) {
-        if (w.l == w.root) return true;
-        --w.l;
-        uint64_t m = lstk_get(lds, ln, w, w.l);
-        unlink(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.l);
-        m &= m - 1;
-        if (m != 0) {
-            lstk_put(lds, ln, w, w.l, m);
-            link(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), w.l);
-            ++w.l;
-            return false;
-        }
     }
 }
 
@@ -201,17 +216,18 @@ __device__ __forceinline__ void lflush(const LunnonArgs &a, uint32_t lane, const
 // LDS: [tables][maps][per wave: b rows | stack].
 __device__ __forceinline__ uint32_t tables_span(uint32_t bytes) { return (bytes + 15u) & ~15u; }
 
-__device__ __forceinline__ LLane lmake_lane(uint32_t tab_span, uint32_t levels) {
+template <typename Stack>
+__device__ __forceinline__ Lane lmake_lane(uint32_t tab_span, uint32_t levels) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t wbase = tab_span + kMapBytes + wave * (kRowBytes + levels * 512u);
-    LLane ln;
-    ln.bl = wbase + lane * 4u;
-    ln.sl = wbase + kRowBytes + lane * 8u;
+    const uint32_t wbase = tab_span + kMapBytes + wave * (kRowBytes + levels * Stack::kSlotBytes);
+    Lane ln;
+    ln.cl = row_lane_base(wbase, lane);
+    ln.sl = wbase + kRowBytes + lane * Stack::kLaneBytes;
+    ln.base = 0;
     return ln;
 }
 
 __device__ __forceinline__ void lstage(const LunnonArgs &a, uint8_t *lds, uint32_t tab_span) {
-    // Tables (byte-granular copy, 4 bytes per thread and step) and map descriptors.
     const uint32_t words = (a.tables_bytes + 3u) / 4u;
     for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) {
         uint32_t v = 0;
@@ -226,22 +242,74 @@ __device__ __forceinline__ void lstage(const LunnonArgs &a, uint8_t *lds, uint32
     __syncthreads();
 }
 
+// Split the lane's remaining work into the spill queue (see sym_kernels.hip).
+template <typename Stack>
+__device__ __forceinline__ void lspill_all(const LunnonArgs &a, uint8_t *lds, const Lane &ln, uint32_t lane,
+                                           LWork &w, LTotals &tot) {
+    bool pend = w.have;
+    uint32_t t = w.l;
+    uint64_t mask = 0;
+    uint32_t floor = w.floor;
+    bool emit = pend;
+    while (__any(pend)) {
+        uint32_t cnt = 0;
+        const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
+        if (emit) {
+            if (slot < a.spill_cap) {
+                WorkItem *dst = a.spill + slot;
+                row_store(lds, ln.cl, dst->state);
+                dst->mask = mask;
+                dst->origin = w.origin;
+                dst->mult = static_cast<uint16_t>(w.n | (floor << 8));
+                dst->level = static_cast<uint8_t>(t);
+                dst->map = static_cast<uint8_t>(w.map);
+            } else {
+                atomicOr(a.acc + ACC_ERROR, 1ull);
+            }
+        }
+        if (cnt != 0 && lane == 0) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
+        emit = false;
+        if (pend) {
+            if (t == w.root) {
+                pend = false;
+            } else {
+                --t;
+                const typename Stack::Entry e = Stack::get(lds, ln, t);
+                const uint32_t j = Stack::child(e);
+                unlink(lds, ln, j, t);
+                bool unknown = false;
+                mask = Stack::rest(e, unknown);
+                floor = 0;
+                if (unknown) {
+                    mask = 0;
+                    floor = j + 1;
+                }
+                emit = mask != 0 || floor != 0;
+            }
+        }
+    }
+    if (w.have) {
+        lretire(a, w, tot);
+        w.have = false;
+    }
+}
+
 }  // namespace
 
-template <bool EV>
+template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t span = tables_span(a.tables_bytes);
     lstage(a, lds, span);
     const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
-    const LLane ln = lmake_lane(span, a.levels);
+    Lane ln = lmake_lane<Stack>(span, a.levels);
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     LTotals tot;
     LWork w;
     w.have = false;
-    if (idx < a.num_items) lload_item(lds, ln, a.items + idx, maps, w);
+    if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
     while (w.have) {
-        if (ldfs_step<EV>(lds, ln, w)) {
+        if (ldfs_step<Stack, EV>(lds, ln, w)) {
             lretire(a, w, tot);
             w.have = false;
         }
@@ -249,13 +317,13 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     lflush(a, threadIdx.x & 63u, tot);
 }
 
-template <bool EV>
+template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t span = tables_span(a.tables_bytes);
     lstage(a, lds, span);
     const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
-    const LLane ln = lmake_lane(span, a.levels);
+    Lane ln = lmake_lane<Stack>(span, a.levels);
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
 
@@ -277,7 +345,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                 if (got + want >= a.num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < a.num_items) lload_item(lds, ln, a.items + idx, maps, w);
+                    if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
                 }
             } else {
                 drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.num_items;
@@ -286,7 +354,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
         if (!__any(w.have)) break;
 
         for (uint32_t s = 0; s < a.budget && w.have; ++s) {
-            if (ldfs_step<EV>(lds, ln, w)) {
+            if (ldfs_step<Stack, EV>(lds, ln, w)) {
                 lretire(a, w, tot);
                 w.have = false;
             }
@@ -295,44 +363,15 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
         if (drained) {
             tail_left = tail_left > a.budget ? tail_left - a.budget : 0;
             if (tail_left == 0) {
-                bool pend = w.have;
-                uint32_t t = w.l;
-                uint64_t mask = 0;
-                bool emit = pend;
-                while (__any(pend)) {
-                    uint32_t cnt = 0;
-                    const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
-                    if (emit) {
-                        if (slot < a.spill_cap) {
-                            WorkItem *dst = a.spill + slot;
-                            lds_to_state(lds, ln.bl, dst->state);
-                            dst->mask = mask;
-                            dst->origin = w.origin;
-                            dst->mult = static_cast<uint16_t>(w.n);
-                            dst->level = static_cast<uint8_t>(t);
-                            dst->map = static_cast<uint8_t>(w.map);
-                        } else {
-                            atomicOr(a.acc + ACC_ERROR, 1ull);
-                        }
-                    }
-                    if (cnt != 0 && lane == 0) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
-                    emit = false;
-                    if (pend) {
-                        if (t == w.root) {
-                            pend = false;
-                        } else {
-                            --t;
-                            const uint64_t m = lstk_get(lds, ln, w, t);
-                            unlink(lds, ln, static_cast<uint32_t>(__builtin_ctzll(m)), t);
-                            mask = m & (m - 1);
-                            emit = mask != 0;
-                        }
+                // Progress guarantee: every lane completes one counted gap
+                // computation of its item before its remaining work is split.
+                while (__any(w.have && !w.progressed)) {
+                    if (w.have && !w.progressed && ldfs_step<Stack, EV>(lds, ln, w)) {
+                        lretire(a, w, tot);
+                        w.have = false;
                     }
                 }
-                if (w.have) {
-                    lretire(a, w, tot);
-                    w.have = false;
-                }
+                lspill_all<Stack>(a, lds, ln, lane, w, tot);
                 break;
             }
         }
@@ -340,21 +379,30 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     lflush(a, lane, tot);
 }
 
-uint32_t lunnon_lds_bytes(uint32_t tables_bytes, uint32_t levels, uint32_t block) {
-    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (kRowBytes + levels * 512u);
+namespace {
+template <typename Stack>
+const void *lpick(bool persistent, bool events) {
+    if (persistent) return events ? reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, true>)
+                                  : reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, false>);
+    return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, true>)
+                  : reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, false>);
+}
+}  // namespace
+
+uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block) {
+    const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
+    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (kRowBytes + levels * slot);
 }
 
-const void *lunnon_kernel_ptr(bool persistent, bool events) {
-    if (persistent) return events ? reinterpret_cast<const void *>(&lunnon_persistent_kernel<true>)
-                                  : reinterpret_cast<const void *>(&lunnon_persistent_kernel<false>);
-    return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<true>)
-                  : reinterpret_cast<const void *>(&lunnon_static_kernel<false>);
+// core: 1 = wide u64 stack, otherwise the compact 16-bit stack.
+const void *lunnon_kernel_ptr(int core, bool persistent, bool events) {
+    return core == 1 ? lpick<WideStack>(persistent, events) : lpick<CompactStack>(persistent, events);
 }
 
-hipError_t launch_lunnon(const LunnonArgs &a, bool persistent, uint32_t grid, uint32_t block, uint32_t lds_bytes,
-                         bool events, hipStream_t s) {
+hipError_t launch_lunnon(const LunnonArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,
+                         uint32_t lds_bytes, bool events, hipStream_t s) {
     void *args[] = {const_cast<LunnonArgs *>(&a)};
-    return hipLaunchKernel(lunnon_kernel_ptr(persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
+    return hipLaunchKernel(lunnon_kernel_ptr(core, persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
 }
 
 }  // namespace mf

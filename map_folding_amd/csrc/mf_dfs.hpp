@@ -1,0 +1,124 @@
+// mf_dfs.hpp -- per-lane DFS state in LDS, shared by the sym and Lunnon kernels.
+//
+// Each lane owns one 64-entry byte row (sym: cycle successor per cell;
+// Lunnon: "below" link per leaf) and one stack with an entry per DFS level
+// holding the children of that level's node not yet entered.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mf_wave.hpp"
+
+namespace mf {
+
+constexpr uint32_t kRowBytes = 4096;   // 64 entries x 64 lanes
+
+__device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ? ~0ull : ((1ull << i) - 1ull); }
+
+// Row layout, byte-interleaved: entry c of lane L at byte c*64 + L -- one VALU
+// op per address (the four lanes of a quad share dwords, so lanes of a quad
+// reading different entries can meet in a bank).  The dword-interleaved
+// alternative (dword w of lane L at word w*64 + L: conflict-free, four VALU ops
+// per address) measured 11% slower on the 8x8 depth-36 chunk.
+__device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) { return lane_base + (c << 6); }
+__device__ __forceinline__ uint32_t row_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
+
+__device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = s4[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+    }
+}
+
+__device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+        d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+struct Lane {
+    uint32_t cl;      // LDS byte address of this lane's row entry 0
+    uint32_t sl;      // LDS byte address of this lane's stack slot 0
+    uint32_t base;    // level held by stack slot 0
+};
+
+// ---------------------------------------------------------- stack encodings
+// An entry names the current child j of its level's node (ctz of what is
+// left) and the siblings after it.  Children are entered in ascending order,
+// so "the siblings after j" is always every child of the node above j.
+
+// One u64 mask per level.
+struct WideStack {
+    static constexpr uint32_t kSlotBytes = 512;   // u64 x 64 lanes
+    static constexpr uint32_t kLaneBytes = 8;
+    using Entry = uint64_t;
+    static __device__ __forceinline__ uint64_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
+        return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+    }
+    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t e) {
+        *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = e;
+    }
+    static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
+    static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
+    // 0 = no sibling after the current child, 1 = advance() gives it.
+    static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
+    static __device__ __forceinline__ uint64_t advance(uint64_t e) { return e & (e - 1); }
+    // The siblings after the current child as a mask (for splitting); never
+    // unknown for this encoding.
+    static __device__ __forceinline__ uint64_t rest(uint64_t e, bool &unknown) {
+        unknown = false;
+        return e & (e - 1);
+    }
+};
+
+// 16 bits per level: bits 0-5 current child j, bits 6-11 next sibling,
+// bits 12-13 state: 0 = j is the last child, 1 = the next sibling is the
+// last, 2 = more follow the next sibling, 3 = more follow j but are not
+// recorded (the node is recomputed to find them).
+struct CompactStack {
+    static constexpr uint32_t kSlotBytes = 128;   // u16 x 64 lanes
+    static constexpr uint32_t kLaneBytes = 2;
+    using Entry = uint32_t;
+    static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
+        return *reinterpret_cast<const uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+    }
+    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
+        *reinterpret_cast<uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = static_cast<uint16_t>(e);
+    }
+    static __device__ __forceinline__ uint32_t encode(uint64_t g) {
+        const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(g));
+        const uint64_t r = g & (g - 1);
+        if (r == 0) return j;
+        const uint32_t j2 = static_cast<uint32_t>(__builtin_ctzll(r));
+        const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
+        return j | (j2 << 6) | (state << 12);
+    }
+    static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & 63u; }
+    // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.
+    static __device__ __forceinline__ uint32_t more(uint32_t e) {
+        const uint32_t s = e >> 12;
+        return s == 0 ? 0u : (s == 3 ? 2u : 1u);
+    }
+    static __device__ __forceinline__ uint32_t advance(uint32_t e) {
+        const uint32_t j2 = (e >> 6) & 63u;
+        return j2 | ((e >> 12) == 2u ? (3u << 12) : 0u);
+    }
+    static __device__ __forceinline__ uint64_t rest(uint32_t e, bool &unknown) {
+        const uint32_t s = e >> 12;
+        unknown = s >= 2u;
+        return s == 1u ? (1ull << ((e >> 6) & 63u)) : 0ull;
+    }
+};
+
+}  // namespace mf

@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include "mf_device.hpp"
+#include "mf_dfs.hpp"
 #include "mf_wave.hpp"
 
 namespace mf {
@@ -48,54 +49,9 @@ namespace {
 constexpr uint32_t kPartnerOff = 0;      // 64 words: byte p = partner on page p (n2 = none)
 constexpr uint32_t kPageOff = 256;       // the same, page-major: byte p*64 + c
 constexpr uint32_t kTableBytes = 512;
-constexpr uint32_t kCycleBytes = 4096;   // 16 words x 64 lanes
+constexpr uint32_t kCycleBytes = kRowBytes;
 
-// Cycle layout.  Byte-interleaved (default): cell c of lane L at byte
-// c*64 + L -- one VALU op per address; the four lanes of a quad share
-// dwords, so lanes of a quad looking up different cells can conflict.
-// Word-interleaved (MF_SYM_BYTE_LAYOUT=0): dword w of lane L holds cells
-// 4w..4w+3 at word w*64 + L, so lanes never share a bank, at four VALU ops
-// per address.  Measured on the 8x8 depth-36 chunk: byte 1.63 s, word 1.82 s.
-#ifndef MF_SYM_BYTE_LAYOUT
-#define MF_SYM_BYTE_LAYOUT 1
-#endif
-#if MF_SYM_BYTE_LAYOUT
-__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return lane_base + (cell << 6); }
-__device__ __forceinline__ uint32_t cyc_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
-__device__ __forceinline__ void cyc_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const uint4 v = s4[q];
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int k = 0; k < 16; ++k) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
-    }
-}
-__device__ __forceinline__ void cyc_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
-    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 16; ++k) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
-        d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-}
-#else
-__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_addr(lane_base, cell); }
-__device__ __forceinline__ uint32_t cyc_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane * 4u; }
-__device__ __forceinline__ void cyc_load(uint8_t *lds, uint32_t lb, const uint8_t *src) { state_to_lds(lds, lb, src); }
-__device__ __forceinline__ void cyc_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) { lds_to_state(lds, lb, dst); }
-#endif
-
-__device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ? ~0ull : ((1ull << i) - 1ull); }
-
-struct Lane {
-    uint32_t cl;      // LDS byte address of this lane's cycle (dword 0)
-    uint32_t sl;      // LDS byte address of this lane's stack slot 0
-    uint32_t base;    // level held by stack slot 0
-};
+__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_at(lane_base, cell); }
 
 // Insert cell i after cell j (iter_good, sym_fold.cpp:210-213).
 __device__ __forceinline__ void place(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
@@ -107,70 +63,6 @@ __device__ __forceinline__ void place(uint8_t *lds, const Lane &ln, uint32_t j, 
 __device__ __forceinline__ void unplace(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
     lds[cyc_addr(ln.cl, j)] = lds[cyc_addr(ln.cl, i)];
 }
-
-// ---------------------------------------------------------- stack encodings
-
-// A level's entry: the current child j (ctz of what is left) and the
-// siblings after it.
-struct WideStack {
-    static constexpr uint32_t kSlotBytes = 512;   // u64 x 64 lanes
-    using Entry = uint64_t;
-    static __device__ __forceinline__ uint64_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-        return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
-    }
-    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t e) {
-        *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = e;
-    }
-    static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
-    static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
-    // Siblings after the current child: 0 = none, 1 = next() is the next
-    // child and advance() the new entry.  (Never "recompute".)
-    static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
-    static __device__ __forceinline__ uint64_t advance(uint64_t e) { return e & (e - 1); }
-    // The siblings left as a mask (for splitting); `unknown` set when the
-    // rest must be recomputed above child(e).
-    static __device__ __forceinline__ uint64_t rest(uint64_t e, bool &unknown) {
-        unknown = false;
-        return e & (e - 1);
-    }
-};
-
-// bits 0-5 current child j, bits 6-11 next sibling, bits 12-13 state:
-// 0 = j is the last child, 1 = the next sibling is the last, 2 = more follow
-// the next sibling, 3 = more follow j but are not recorded (recompute).
-struct CompactStack {
-    static constexpr uint32_t kSlotBytes = 128;   // u16 x 64 lanes
-    using Entry = uint32_t;
-    static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-        return *reinterpret_cast<const uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
-    }
-    static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
-        *reinterpret_cast<uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = static_cast<uint16_t>(e);
-    }
-    static __device__ __forceinline__ uint32_t encode(uint64_t g) {
-        const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(g));
-        const uint64_t r = g & (g - 1);
-        if (r == 0) return j;
-        const uint32_t j2 = static_cast<uint32_t>(__builtin_ctzll(r));
-        const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
-        return j | (j2 << 6) | (state << 12);
-    }
-    static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & 63u; }
-    // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.
-    static __device__ __forceinline__ uint32_t more(uint32_t e) {
-        const uint32_t s = e >> 12;
-        return s == 0 ? 0u : (s == 3 ? 2u : 1u);
-    }
-    static __device__ __forceinline__ uint32_t advance(uint32_t e) {
-        const uint32_t j2 = (e >> 6) & 63u;
-        return j2 | ((e >> 12) == 2u ? (3u << 12) : 0u);
-    }
-    static __device__ __forceinline__ uint64_t rest(uint32_t e, bool &unknown) {
-        const uint32_t s = e >> 12;
-        unknown = s >= 2u;
-        return s == 1u ? (1ull << ((e >> 6) & 63u)) : 0ull;
-    }
-};
 
 struct Work {
     uint32_t i;        // level of the node to expand next
@@ -274,7 +166,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
 // map == f > 0: recompute node `level` and enter its children above f-1.
 __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it,
                                                  Work &w) {
-    cyc_load(lds, ln.cl, it->state);
+    row_load(lds, ln.cl, it->state);
     w.origin = it->origin;
     w.mult = it->mult;
     w.root = it->level;
@@ -337,8 +229,8 @@ __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) 
     const uint32_t wave_bytes = kCycleBytes + levels * Stack::kSlotBytes;
     const uint32_t wbase = kTableBytes + wave * wave_bytes;
     Lane ln;
-    ln.cl = cyc_lane_base(wbase, lane);
-    ln.sl = wbase + kCycleBytes + lane * static_cast<uint32_t>(sizeof(typename Stack::Entry) == 8 ? 8 : 2);
+    ln.cl = row_lane_base(wbase, lane);
+    ln.sl = wbase + kCycleBytes + lane * Stack::kLaneBytes;
     ln.base = base_level;
     return ln;
 }
@@ -375,7 +267,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         if (emit) {
             if (slot < a.spill_cap) {
                 WorkItem *dst = a.spill + slot;
-                cyc_store(lds, ln.cl, dst->state);
+                row_store(lds, ln.cl, dst->state);
                 dst->mask = mask;
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.mult);
