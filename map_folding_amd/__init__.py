@@ -28,6 +28,7 @@ __all__ = [
     "MF_SCHED_STATIC", "MF_SCHED_RESPLIT", "MFError", "MFResult", "SolveOptions", "init",
     "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
     "sym_cases", "sym_case_total", "sym_solve", "sym_fold", "SymFrontier", "LunnonFrontier",
+    "LunnonBatchFrontier",
     "declared_symbols",
 ]
 

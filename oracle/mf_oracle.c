@@ -847,3 +847,64 @@ int orc_sym_answer(int n, int expanded, uint64_t start, uint64_t max_cases, int 
     if (st) *st = local;
     return 0;
 }
+
+/* ---- sym_kernel.c semantics (mfah/wasm/sym_kernel.c:212-230) ------------- */
+
+/* Paths from level i to level stop: recurse_count of sym_kernel.c, one node
+   per call including the calls that reach stop. */
+static uint64_t orc_count_to(const orc_sym_problem *pr, uint8_t *cycle, int i, int stop, uint64_t *nodes,
+                             orc_sym_stats *st) {
+    ++*nodes;
+    if (i == stop) return 1;
+    uint64_t good = orc_get_good(pr, cycle, i, st);
+    uint64_t out = 0;
+    while (good) {
+        int j = __builtin_ctzll(good);
+        good &= good - 1;
+        uint8_t next = cycle[j];
+        cycle[i] = next;
+        cycle[j] = (uint8_t)i;
+        out += orc_count_to(pr, cycle, i + 1, stop, nodes, st);
+        cycle[j] = next;
+    }
+    return out;
+}
+
+int orc_sym_count_to(int n, const uint8_t *cycle_in, int start, int stop, uint64_t *count, uint64_t *nodes) {
+    if (n < 2 || n > 8) return 1;
+    orc_sym_problem pr;
+    orc_sym_setup(&pr, n);
+    if (start < 1 || start > stop || stop > pr.n2) return 2;
+    uint8_t cycle[SYM_MAX];
+    memset(cycle, 0, sizeof(cycle));
+    memcpy(cycle, cycle_in, (size_t)start);
+    orc_sym_stats st;
+    memset(&st, 0, sizeof(st));
+    *nodes = 0;
+    *count = orc_count_to(&pr, cycle, start, stop, nodes, &st);
+    return 0;
+}
+
+/* All cycles at `depth` grown from the initial cycle 0->1->2->0 at cell 3, in
+   DFS order (the prefixes a volunteer work unit carries). */
+int64_t orc_sym_prefixes(int n, int depth, uint8_t *out, int64_t max_out) {
+    if (n < 3 || n > 8) return -1;
+    orc_sym_problem pr;
+    orc_sym_setup(&pr, n);
+    if (depth < 3 || depth > pr.n2) return -2;
+    uint8_t cycle[SYM_MAX];
+    memset(cycle, 0, sizeof(cycle));
+    cycle[0] = 1;
+    cycle[1] = 2;
+    cycle[2] = 0;
+    orc_cycle_vec v = {0, 0, 0};
+    orc_sym_stats st;
+    memset(&st, 0, sizeof(st));
+    orc_generate(&pr, cycle, 3, depth, &v, &st);
+    int64_t k = v.len < max_out ? v.len : max_out;
+    if (out)
+        for (int64_t q = 0; q < k; q++) memcpy(out + q * SYM_MAX, v.v[q].cyc, SYM_MAX);
+    int64_t total = v.len;
+    free(v.v);
+    return total;
+}

@@ -229,3 +229,30 @@ def sym_answer(n, expanded=24, start=0, max_cases=0, threads=None):
     if rc:
         raise ValueError(f"answer failed rc={rc}")
     return int(tot[0]) + (int(tot[1]) << 64), st.as_dict()
+
+
+def sym_count_to(n, cycle, start, stop):
+    """sym_kernel.c recurse_count (mfah/wasm/sym_kernel.c:212-230): (paths start -> stop, nodes)."""
+    L = lib()
+    L.orc_sym_count_to.argtypes = [C.c_int, C.POINTER(C.c_uint8), C.c_int, C.c_int,
+                                   C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    cyc = np.zeros(64, dtype=np.uint8)
+    cyc[:start] = np.asarray(cycle, dtype=np.uint8)[:start]
+    c, nd = C.c_uint64(), C.c_uint64()
+    rc = L.orc_sym_count_to(n, _ptr(cyc, C.c_uint8), start, stop, C.byref(c), C.byref(nd))
+    if rc:
+        raise ValueError(f"bad request rc={rc}")
+    return int(c.value), int(nd.value)
+
+
+def sym_prefixes(n, depth):
+    """All cycles at `depth` grown from the initial cycle 0->1->2->0 (uint8 [k, 64])."""
+    L = lib()
+    L.orc_sym_prefixes.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint8), C.c_int64]
+    L.orc_sym_prefixes.restype = C.c_int64
+    k = L.orc_sym_prefixes(n, depth, None, 0)
+    if k < 0:
+        raise ValueError("bad request")
+    out = np.zeros((k, 64), dtype=np.uint8)
+    L.orc_sym_prefixes(n, depth, _ptr(out, C.c_uint8), k)
+    return out

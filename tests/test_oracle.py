@@ -179,3 +179,21 @@ def test_sym_cases_window_matches_stream(oracle):
     cyc, mul = oracle.sym_cases(7, 24, 0, 400)
     c2, m2 = oracle.sym_cases(7, 24, 123, 200)
     assert np.array_equal(cyc[123:323], c2) and np.array_equal(mul[123:323], m2)
+
+
+def test_sym_prefix_count_to_reproduces_squares(oracle, golden):
+    """The sym_kernel.c-style split (all prefix cycles from the fixed 0->1->2->0
+    start, each counted to n^2) sums to the published square totals:
+    total = 2 * n^2 * (number of full paths) (mfah/wasm/sym_kernel.c:240-277)."""
+    for n, depth in ((4, 10), (5, 14)):
+        pre = oracle.sym_prefixes(n, depth)
+        paths = sum(oracle.sym_count_to(n, c, depth, n * n)[0] for c in pre)
+        assert 2 * n * n * paths == golden["square_totals"][str(n)]
+
+
+def test_sym_count_to_intermediate_stop_counts_prefixes(oracle):
+    """Counting from depth 12 to stop 16 counts exactly the depth-16 cycles the
+    prefix generator enumerates (two independent walks of the same tree)."""
+    pre = oracle.sym_prefixes(5, 12)
+    mid = sum(oracle.sym_count_to(5, c, 12, 16)[0] for c in pre)
+    assert mid == oracle.sym_prefixes(5, 16).shape[0]
