@@ -12,6 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmapfold.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mapfold.h")
+HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mapfold_cost.h")]
 
 MF_OK = 0
 MF_ERR_ARG = 1
@@ -53,6 +54,18 @@ class MFSolveOptions(C.Structure):
     ]
 
 
+class MFCostTrainOptions(C.Structure):
+    _fields_ = [
+        ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+        ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64),
+    ]
+
+
+MF_COST_FEATURES = 80
+MF_COST_HIDDEN = 128
+MF_COST_PARAMS = 43521
+
+
 class MFError(RuntimeError):
     def __init__(self, status: int, where: str, message: str):
         super().__init__(f"{where}: status {status} ({message})")
@@ -72,7 +85,7 @@ def lib() -> C.CDLL:
                           "or `make -C map_folding_amd/csrc`")
     L = C.CDLL(LIB_PATH)
     P = C.POINTER
-    i32, i64, u32, u64, u8 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_uint8
+    i32, i64, u32, u64, u8, f32 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_uint8, C.c_float
     sig = {
         "mf_status_string": (C.c_char_p, [C.c_int]),
         "mf_last_error": (C.c_char_p, []),
@@ -101,6 +114,18 @@ def lib() -> C.CDLL:
         "mf_wasm_max_cases": (u32, []),
         "mf_wasm_solve_cases": (u32, [u32, u32, u32, u32]),
         "mf_frontier_destroy": (None, [C.c_void_p]),
+        # learned cost model (include/mapfold_cost.h)
+        "mf_cost_create": (C.c_int, [u64, P(C.c_void_p)]),
+        "mf_cost_destroy": (None, [C.c_void_p]),
+        "mf_cost_get_params": (C.c_int, [C.c_void_p, P(f32)]),
+        "mf_cost_set_params": (C.c_int, [C.c_void_p, P(f32)]),
+        "mf_cost_features_sym": (C.c_int, [i32, i32, P(u8), i64, P(f32)]),
+        "mf_cost_features_lunnon": (C.c_int, [P(i32), i32, i32, P(u8), i64, P(f32)]),
+        "mf_cost_gradient": (C.c_int, [C.c_void_p, P(f32), P(f32), i64, P(f32), P(f32)]),
+        "mf_cost_train": (C.c_int, [C.c_void_p, P(f32), P(f32), i64, P(MFCostTrainOptions), P(f32)]),
+        "mf_sym_estimate_work": (C.c_int, [i32, i32, P(u8), i64, i32, P(u64)]),
+        "mf_cost_predict": (C.c_int, [C.c_void_p, P(f32), i64, P(f32)]),
+        "mf_cost_predict_fp32": (C.c_int, [C.c_void_p, P(f32), i64, P(f32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -116,8 +141,8 @@ def check(status: int, where: str) -> None:
 
 
 def declared_symbols() -> list[str]:
-    """Function names declared in include/mapfold.h."""
+    """Function names declared in include/mapfold.h and include/mapfold_cost.h."""
     import re
-    text = open(HEADER_PATH).read()
+    text = "\n".join(open(p).read() for p in HEADER_PATHS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(mf_[a-z0-9_]+)\s*\(", text)))

@@ -436,6 +436,18 @@ WorkItem lunnon_item(const uint8_t *b, int depth, int n, uint32_t origin, int ma
 
 }  // namespace
 
+// Shared with the other translation units of the library (cost model).
+namespace mf {
+int lib_error(int status, const char *msg) { return fail(status, "%s", msg); }
+int lib_context(hipStream_t *stream, int *cus) {
+    const int rc = require_init();
+    if (rc != MF_OK) return rc;
+    *stream = g.stream;
+    *cus = g.cus;
+    return MF_OK;
+}
+}  // namespace mf
+
 // ====================================================================== C ABI
 
 extern "C" {
