@@ -9,10 +9,16 @@ chunk total is checked bit-exactly (against the published/reference golden
 for the default chunk) on every timed step.
 
 Prints ONE JSON line.  `value` is DFS nodes/s (node = one get_good call) of
-the batched persistent re-split schedule; the plain static-frontier kernel
-(one lane per case, one launch) is timed beside it on the chunk's first
---plain-cases cases (it is bounded by its longest case: minutes on the whole
-chunk).  Other configurations: --workload sym7 | lunnon6 | rect64.
+the batched persistent re-split schedule fed by the learned cost model: the
+model is trained on the GPU (pairs logged by the GPU DFS on case windows
+disjoint from the timed chunk), predicts every case, and cases predicted large
+are split into subtrees that are queued largest-first (outside the timed
+region, like the case generation; both times are reported).  The same
+schedule without the model, and the plain static-frontier kernel (one lane
+per case, one launch) on the chunk's first --plain-cases cases (it is bounded
+by its longest case: minutes on the whole chunk), are timed beside it, the
+latter also on learned and lookahead-proxy splits.  Other configurations:
+--workload sym7 | lunnon6 | rect64.
 
     python bench.py [--steps K] [--warmup W] [--workload W] [--start S] [--cases C]
                     [--no-plain] [--no-cpu] [--cpu-cases M] [--plain-cases P]
@@ -162,6 +168,8 @@ def run_lunnon(args, mf, rl):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s", "frac": achieved / peak,
                      "traffic": None, "ops_per_step": ops,
                      "ops_model": "10*gapcomps + 8*walk_steps + 6*candidates + 8*inserts + 2*completions"},
+        "no_model": no_model,
+        "learned": learned,
         "plain": plain, "cpu_baseline": cpu, "frontier_gen_s": gen_s,
     }
 
@@ -186,6 +194,12 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--core", type=int, default=0,
                     help="sym DFS stack: 0 default, 1 one u64 mask per level, 2 compact 16-bit entries")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="sym: time the persistent schedule without the learned pre-split")
+    ap.add_argument("--cost-params", default="",
+                    help="sym: load cost-model parameters (.npy) instead of training them on the GPU")
+    ap.add_argument("--split-threshold", type=float, default=15.0,
+                    help="sym: split cases whose predicted log2(1+nodes) exceeds this")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -218,7 +232,8 @@ def main():
     batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
                               tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
     # Exact event counts (schedule-independent) for the algorithmic-work figure.
-    res, _, _ = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core))
+    res, _, case_nodes = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True,
+                                                     core=args.core), per_item=True)
     nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
     total = res.count * n * n
     golden = golden0 if (args.start, args.cases) == (start0, cases0) else None
@@ -227,16 +242,62 @@ def main():
     log(f"counting run: total={total} nodes={nodes} walks={walks} steps={steps} rounds={res.rounds} "
         f"kernel_ms={res.kernel_ms:.1f}")
 
+    # The persistent schedule without the learned part (one step, for reference).
+    r0, _, _ = front.solve(batched)
+    no_model = {"ms": r0.total_ms, "kernel_ms": r0.kernel_ms, "rounds": r0.rounds,
+                "nodes_per_s": nodes / (r0.total_ms / 1e3)}
+    log(f"persistent, no model: {r0.total_ms:.1f} ms (kernels {r0.kernel_ms:.1f} ms, {r0.rounds} rounds)")
+
+    # Learned part: train the cost model on the GPU (pairs logged by the GPU DFS
+    # on windows disjoint from this chunk), predict every case, split the ones
+    # predicted large into subtrees, order the pieces largest-first.
+    learned = None
+    timed, timed_opts, gpu_nodes = front, batched, nodes
+    if not args.no_learned:
+        from map_folding_amd import cost
+        t1 = time.time()
+        if args.cost_params:
+            model = cost.CostModel(seed=0)
+            model.load(args.cost_params)
+            pairs, losses = None, []
+        else:
+            model, losses, pairs = cost.train_default(seed=0)
+        train_s = time.time() - t1
+        pred = model.predict(cost.features_sym(n, level, cyc))
+        y = cost.node_target(case_nodes)
+        proxy = cost.estimate_work(n, level, cyc, 8)
+        t1 = time.time()
+        pieces = cost.learned_split(model, n, level, cyc, mul, args.split_threshold)
+        split_s = time.time() - t1
+        timed = pieces.frontier(n)
+        timed_opts = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
+                                     tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
+        rl_ev, pc_l, _ = timed.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True,
+                                                     core=args.core), per_item=True)
+        if rl_ev.count * n * n != total or rl_ev.nodes + pieces.host_nodes != nodes:
+            raise SystemExit("learned pieces disagree with the case frontier")
+        gpu_nodes = rl_ev.nodes
+        walks, steps = rl_ev.walks, rl_ev.steps
+        learned = {"pairs": pairs, "train_s": train_s, "final_loss": float(losses[-1]) if len(losses) else None,
+                   "rmse_log2_nodes": float(np.sqrt(np.mean((pred - y) ** 2))),
+                   "spearman": cost.spearman(pred, case_nodes),
+                   "spearman_lookahead8_proxy": cost.spearman(proxy, case_nodes),
+                   "split_threshold": args.split_threshold, "pieces": len(pieces), "host_nodes": pieces.host_nodes,
+                   "split_s": split_s}
+        log(f"cost model: {train_s:.1f}s to train, rmse {learned['rmse_log2_nodes']:.2f} (log2 nodes), "
+            f"spearman {learned['spearman']:.3f} (lookahead-8 proxy {learned['spearman_lookahead8_proxy']:.3f}); "
+            f"{len(pieces)} pieces in {split_s:.2f}s")
+
     for w in range(args.warmup):
-        r, _, _ = front.solve(batched)
+        r, _, _ = timed.solve(timed_opts)
         log(f"warmup {w}: {r.total_ms:.1f} ms")
 
     wall, kern, span, rounds = [], [], [], []
     for k in range(args.steps):
         t1 = time.perf_counter()
-        r, _, _ = front.solve(batched)
+        r, _, _ = timed.solve(timed_opts)
         wall.append(time.perf_counter() - t1)
-        if r.count * n * n != total or r.nodes != nodes:
+        if r.count * n * n != total or r.nodes != gpu_nodes:
             raise SystemExit("a timed step disagreed with the counting run")
         kern.append(r.kernel_ms / 1e3)
         span.append(r.span_ms / 1e3)
@@ -244,10 +305,10 @@ def main():
         log(f"step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}")
 
     ms_per_step = 1e3 * sum(wall) / len(wall)
-    node_rate = nodes / (ms_per_step / 1e3)
+    node_rate = gpu_nodes / (ms_per_step / 1e3)
     case_rate = cyc.shape[0] / (ms_per_step / 1e3)
     kernel_s = sum(kern) / len(kern)
-    ops = rl.sym_ops(nodes, walks, steps)
+    ops = rl.sym_ops(gpu_nodes, walks, steps)
     achieved = ops / kernel_s / 1e12
     peak = rl.int32_peak_ops(cus or 256) / 1e12
 
@@ -276,6 +337,21 @@ def main():
                  "speedup_batched_vs_plain": pw / bw}
         log(f"plain static: {pw*1e3:.1f} ms; batched on the same cases: {bw*1e3:.1f} ms")
         sub.close()
+        if learned is not None:
+            # The same static one-lane-per-item kernel on split pieces: learned
+            # split versus the reference's lookahead proxy (estimate_work, depth 8).
+            for name, pcs in (("learned_static", cost.learned_split(model, n, level, cyc[:m], mul[:m], 12.0)),
+                              ("lookahead_static", cost.lookahead_split(n, level, cyc[:m], mul[:m], 5.0, depth=8))):
+                fr = pcs.frontier(n)
+                t1 = time.perf_counter()
+                rq, _, _ = fr.solve(mf.SolveOptions(schedule=mf.MF_SCHED_STATIC, count_events=False,
+                                                    core=args.core))
+                qw = time.perf_counter() - t1
+                if rq.count != rs.count:
+                    raise SystemExit(f"{name} disagreed with the re-split schedule")
+                plain[name] = {"pieces": len(pcs), "ms": qw * 1e3, "speedup_vs_plain": pw / qw}
+                log(f"{name}: {len(pcs)} pieces, {qw*1e3:.1f} ms")
+                fr.close()
 
     cpu = None
     if not args.no_cpu:
@@ -304,7 +380,9 @@ def main():
         "dtype": "int32",
         "data": "synthetic: deterministic sym_fold case stream (no dataset)",
         "config": {"workload": f"{n}x{n} sym_fold MF_EXPANDED_N={level} chunk [{args.start},{args.start + cyc.shape[0]})",
-                   "cases": int(cyc.shape[0]), "schedule": "persistent re-split", "chunk_total": str(total),
+                   "cases": int(cyc.shape[0]),
+                   "schedule": "persistent re-split" + ("" if learned is None else " + learned pre-split/order"),
+                   "chunk_total": str(total),
                    "bit_exact_vs_golden": golden is not None},
         "cases_per_s": case_rate,
         "nodes": nodes, "page_walks": walks, "walk_steps": steps, "leaves": leaves,
@@ -312,8 +390,11 @@ def main():
         "kernel_ms_per_step": kernel_s * 1e3,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": _traffic(args),
-                     "ops_per_step": ops, "ops_model": "16*nodes + 6*page_walks + 8*walk_steps (int32)",
+                     "ops_per_step": ops,
+                     "ops_model": "16*nodes + 6*page_walks + 8*walk_steps (int32), GPU events of the timed step",
                      "peak_model": "int32 VALU: CUs x 64 lanes x 2.4 GHz"},
+        "no_model": no_model,
+        "learned": learned,
         "plain": plain,
         "speedup_vs_plain": plain["speedup_batched_vs_plain"] if plain else None,
         "cpu_baseline": cpu,

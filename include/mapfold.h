@@ -97,7 +97,10 @@ typedef struct {
     int32_t count_events;     /* 1: count walks/steps too (nodes/leaves always counted) */
     int32_t core;             /* sym DFS stack: 0 = default (compact), 1 = one u64 mask per
                                  level, 2 = compact 16-bit entries with recompute */
-    int32_t reserved;
+    int32_t graph;            /* re-split: 0 = default (on), 1 = on, 2 = off.  On: after the
+                                 first round, rounds run as a captured HIP graph of 8 rounds
+                                 (device-side item counts), replayed until a round spills
+                                 nothing; off: one host-driven launch per round. */
     uint64_t queue_capacity;  /* re-split: work items the device queue can hold */
     const uint32_t *order;    /* optional host permutation of item indices (learned order);
                                  NULL = input order.  Never changes any count. */
@@ -167,7 +170,20 @@ int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth,
 /* Batched uneven instances as a resident frontier (see mf_lunnon_batch). */
 int mf_lunnon_batch_frontier_create(const int32_t *dims2, const int32_t *depths, int32_t nmaps,
                                     mf_frontier **out);
-/* Solve a resident frontier; per_count/per_nodes are host arrays (nullable). */
+/* Split frontiers (used by the learned schedule, mapfold_cost.h).
+ * mf_sym_expand: the children at level+1 of each case, in ascending child
+ * order (one level of recurse_generate, sym_fold.cpp:252-265); *written =
+ * total children, MF_ERR_CAPACITY if more than max_out.
+ * mf_sym_pieces_create: a resident frontier of subtrees at mixed levels;
+ * piece s (cycle through cells [0, levels[s])) adds mults[s] x its count to
+ * per-item result origins[s] < num_origins.  The per_count / per_nodes arrays
+ * of mf_frontier_solve then have num_origins entries. */
+int mf_sym_expand(int32_t n, int32_t level, const uint8_t *cycles, int64_t count, uint8_t *out_cycles,
+                  uint32_t *out_parent, int64_t max_out, int64_t *written);
+int mf_sym_pieces_create(int32_t n, const uint8_t *cycles, const uint8_t *levels, const uint32_t *origins,
+                         const uint64_t *mults, int64_t num, int64_t num_origins, mf_frontier **out);
+/* Solve a resident frontier; per_count/per_nodes are host arrays (nullable);
+ * opt->order, if given, must be a permutation of the items. */
 int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt,
                       uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
 /* Solve a resident Lunnon frontier and return each map's exact total. */

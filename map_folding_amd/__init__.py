@@ -21,6 +21,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import _abi
 from ._abi import (MF_SCHED_RESPLIT, MF_SCHED_STATIC, MFError, MFResult, MFSolveOptions, check,
                    declared_symbols, lib)
 
@@ -28,7 +29,7 @@ __all__ = [
     "MF_SCHED_STATIC", "MF_SCHED_RESPLIT", "MFError", "MFResult", "SolveOptions", "init",
     "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
     "sym_cases", "sym_case_total", "sym_solve", "sym_fold", "SymFrontier", "LunnonFrontier",
-    "LunnonBatchFrontier",
+    "LunnonBatchFrontier", "SymPieces", "sym_expand",
     "declared_symbols",
 ]
 
@@ -53,6 +54,7 @@ class SolveOptions:
     core: int = 0                     # sym stack: 0 default, 1 wide u64 masks, 2 compact 16-bit
     queue_capacity: int = 0
     order: np.ndarray | None = None   # permutation of item indices (learned order)
+    graph: int = 0                    # re-split rounds as a HIP graph: 0 default (on), 1 on, 2 off
 
     def to_c(self):
         o = MFSolveOptions()
@@ -64,6 +66,7 @@ class SolveOptions:
         o.count_events = 1 if self.count_events else 0
         o.core = int(self.core)
         o.queue_capacity = int(self.queue_capacity)
+        o.graph = int(self.graph)
         keep = None
         if self.order is not None:
             keep = np.ascontiguousarray(self.order, dtype=np.uint32)
@@ -276,3 +279,39 @@ class LunnonFrontier(_Frontier):
         check(lib().mf_lunnon_frontier_create(dp, len(d), depth, _p(states, C.c_uint8), states.shape[0],
                                               C.byref(h)), "mf_lunnon_frontier_create")
         super().__init__(h, states.shape[0])
+
+
+def sym_expand(n: int, level: int, cycles: np.ndarray):
+    """Children at level+1 of each case (ascending child order): (cycles, parent index)."""
+    cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+    written = C.c_int64()
+    cap = max(1, cycles.shape[0] * 4)
+    while True:
+        out = np.zeros((cap, 64), dtype=np.uint8)
+        par = np.zeros(cap, dtype=np.uint32)
+        rc = lib().mf_sym_expand(n, level, _p(cycles, C.c_uint8), cycles.shape[0], _p(out, C.c_uint8),
+                                 _p(par, C.c_uint32), cap, C.byref(written))
+        if rc == _abi.MF_ERR_CAPACITY and written.value > cap:
+            cap = written.value
+            continue
+        check(rc, "mf_sym_expand")
+        return out[:written.value], par[:written.value]
+
+
+class SymPieces(_Frontier):
+    """Subtrees at mixed levels, each adding its count to one origin case.
+
+    `num` (the length of per-item results) is the number of origins; `items`
+    is the number of resident pieces (what an order permutes)."""
+
+    def __init__(self, n: int, cycles: np.ndarray, levels, origins, mults, num_origins: int):
+        cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+        levels = np.ascontiguousarray(levels, dtype=np.uint8)
+        origins = np.ascontiguousarray(origins, dtype=np.uint32)
+        mults = np.ascontiguousarray(mults, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().mf_sym_pieces_create(n, _p(cycles, C.c_uint8), _p(levels, C.c_uint8), _p(origins, C.c_uint32),
+                                         _p(mults, C.c_uint64), cycles.shape[0], int(num_origins), C.byref(h)),
+              "mf_sym_pieces_create")
+        super().__init__(h, int(num_origins))
+        self.items = cycles.shape[0]

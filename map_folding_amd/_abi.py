@@ -49,7 +49,7 @@ class MFSolveOptions(C.Structure):
     _fields_ = [
         ("schedule", C.c_int32), ("block", C.c_int32), ("waves_per_cu", C.c_int32),
         ("budget", C.c_int32), ("tail", C.c_int32), ("count_events", C.c_int32),
-        ("core", C.c_int32), ("reserved", C.c_int32),
+        ("core", C.c_int32), ("graph", C.c_int32),
         ("queue_capacity", C.c_uint64), ("order", C.POINTER(C.c_uint32)),
     ]
 
@@ -114,6 +114,8 @@ def lib() -> C.CDLL:
         "mf_wasm_max_cases": (u32, []),
         "mf_wasm_solve_cases": (u32, [u32, u32, u32, u32]),
         "mf_frontier_destroy": (None, [C.c_void_p]),
+        "mf_sym_expand": (C.c_int, [i32, i32, P(u8), i64, P(u8), P(u32), i64, P(i64)]),
+        "mf_sym_pieces_create": (C.c_int, [i32, P(u8), P(u8), P(u32), P(u64), i64, i64, P(C.c_void_p)]),
         # learned cost model (include/mapfold_cost.h)
         "mf_cost_create": (C.c_int, [u64, P(C.c_void_p)]),
         "mf_cost_destroy": (None, [C.c_void_p]),

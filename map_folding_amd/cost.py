@@ -200,3 +200,139 @@ def spearman(a, b) -> float:
     ra -= ra.mean()
     rb -= rb.mean()
     return float((ra * rb).sum() / np.sqrt((ra * ra).sum() * (rb * rb).sum()))
+
+
+# ----------------------------------------------------------------------------- split schedules
+
+
+class Pieces:
+    """Subtrees at mixed levels that together cover a list of cases.
+
+    cycles [P, 64], levels [P], origins [P] (case index), mults [P],
+    score [P] (predicted log2 size, or the proxy used to split), host_nodes:
+    nodes expanded on the host while splitting (the parents of split pieces)."""
+
+    def __init__(self, cycles, levels, origins, mults, score, host_nodes, num_origins):
+        self.cycles, self.levels, self.origins, self.mults = cycles, levels, origins, mults
+        self.score, self.host_nodes, self.num_origins = score, host_nodes, num_origins
+
+    def __len__(self):
+        return len(self.levels)
+
+    def frontier(self, n: int, ordered: bool = True):
+        """Resident frontier of the pieces; `ordered` lays them out largest
+        score first once, so solves need no per-call order permutation."""
+        from . import SymPieces
+        if not ordered:
+            return SymPieces(n, self.cycles, self.levels, self.origins, self.mults, self.num_origins)
+        o = self.order()
+        return SymPieces(n, self.cycles[o], self.levels[o], self.origins[o], self.mults[o], self.num_origins)
+
+    def order(self) -> np.ndarray:
+        return learned_order(self.score)
+
+
+def split_cases(n: int, level: int, cycles, mults, score_fn, threshold: float, max_extra: int = 8,
+                max_pieces: int = 4_000_000) -> Pieces:
+    """Replace every case whose score exceeds `threshold` by its children,
+    level by level (at most `max_extra` levels, never past level n^2 - 2),
+    until no piece scores above it or the piece budget is reached.
+
+    score_fn(level, cycles) -> scores (e.g. a model's predicted log2 nodes).
+    Counts are unaffected: a node's subtree is the union of its children's."""
+    from . import sym_expand
+    cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+    K = cycles.shape[0]
+    cur = (cycles, np.arange(K, dtype=np.uint32), np.asarray(mults, dtype=np.uint64))
+    done = []
+    host_nodes = 0
+    total = K
+    lvl = level
+    for extra in range(max_extra + 1):
+        c, o, m = cur
+        if c.shape[0] == 0:
+            break
+        s = np.asarray(score_fn(lvl, c), dtype=np.float32)
+        split = s > threshold
+        if extra == max_extra or lvl + 1 > n * n - 2:
+            split[:] = False
+        keep = ~split
+        done.append((c[keep], np.full(int(keep.sum()), lvl, np.uint8), o[keep], m[keep], s[keep]))
+        if not split.any():
+            break
+        ch, par = sym_expand(n, lvl, c[split])
+        if total - int(split.sum()) + len(ch) > max_pieces:
+            done.append((c[split], np.full(int(split.sum()), lvl, np.uint8), o[split], m[split], s[split]))
+            break
+        host_nodes += int(split.sum())
+        total += len(ch) - int(split.sum())
+        cur = (ch, o[split][par], m[split][par])
+        lvl += 1
+    cat = lambda i: np.concatenate([d[i] for d in done]) if done else np.zeros(0)
+    return Pieces(cat(0).reshape(-1, 64).astype(np.uint8), cat(1).astype(np.uint8), cat(2).astype(np.uint32),
+                  cat(3).astype(np.uint64), cat(4).astype(np.float32), host_nodes, K)
+
+
+def learned_split(model: "CostModel", n: int, level: int, cycles, mults, threshold: float, **kw) -> Pieces:
+    """Split by the network's predicted log2(1 + nodes)."""
+    return split_cases(n, level, cycles, mults, lambda l, c: model.predict(features_sym(n, l, c)), threshold, **kw)
+
+
+def lookahead_split(n: int, level: int, cycles, mults, threshold: float, depth: int = 8, **kw) -> Pieces:
+    """The same splitter driven by the reference's lookahead proxy instead of
+    the network: log2(1 + descendants at level + depth)."""
+    return split_cases(n, level, cycles, mults,
+                       lambda l, c: np.log2(1.0 + estimate_work(n, l, c, depth).astype(np.float64)),
+                       threshold, **kw)
+
+
+# ----------------------------------------------------------------------------- training set
+
+
+def sym_pairs(n: int, level: int, start: int, count: int, stride: int = 1):
+    """(features, log2(1 + nodes), cycles, mults, nodes) for the cases
+    [start, start + count*stride) of the stream, every `stride`-th one, with
+    node counts measured by the GPU DFS (per-item profile of one solve)."""
+    from . import SolveOptions, sym_cases, sym_solve
+    cyc, mul = sym_cases(n, level, start, count * stride)
+    cyc, mul = cyc[::stride], mul[::stride]
+    _, _, nodes = sym_solve(n, level, cyc, mul, options=SolveOptions(count_events=False), per_item=True)
+    return features_sym(n, level, cyc), node_target(nodes), cyc, mul, nodes
+
+
+def lunnon_pairs(dims, depth: int, stride: int = 1):
+    """(features, log2(1 + inserts), states, inserts) of a Lunnon frontier."""
+    from . import SolveOptions, lunnon_frontier, lunnon_solve
+    st = lunnon_frontier(dims, depth)[::stride]
+    _, _, nodes = lunnon_solve(dims, depth, st, options=SolveOptions(count_events=False), per_item=True)
+    return features_lunnon(dims, depth, st), node_target(nodes), st, nodes
+
+
+# Default training windows: the 6x6 depth-18 Lunnon frontier (112,024 states),
+# every 10th 7x7 depth-24 case (43,314), and 3 x 15,000 8x8 depth-36 cases
+# from windows away from [120,000,000, +50,000) -- about 200k pairs.
+DEFAULT_SYM_WINDOWS = ((7, 24, 0, 43314, 10), (8, 36, 0, 15000, 1), (8, 36, 60_000_000, 15000, 1),
+                       (8, 36, 200_000_000, 15000, 1))
+DEFAULT_LUNNON = (([6, 6], 18),)
+
+
+def default_training_set(sym_windows=DEFAULT_SYM_WINDOWS, lunnon=DEFAULT_LUNNON):
+    xs, ys = [], []
+    for n, level, start, count, stride in sym_windows:
+        x, y = sym_pairs(n, level, start, count, stride)[:2]
+        xs.append(x)
+        ys.append(y)
+    for dims, depth in lunnon:
+        x, y = lunnon_pairs(dims, depth)[:2]
+        xs.append(x)
+        ys.append(y)
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def train_default(seed: int = 0, epochs: int = 12, batch: int = 512, lr: float = 1e-3, data=None):
+    """Collect the default pairs with the GPU DFS and train a model on the GPU.
+    Returns (model, per-epoch losses, number of pairs)."""
+    x, y = default_training_set() if data is None else data
+    m = CostModel(seed=seed)
+    losses = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return m, losses, len(y)

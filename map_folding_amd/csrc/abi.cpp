@@ -27,6 +27,7 @@ hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t
                          uint32_t lds, bool events, hipStream_t);
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block);
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
+hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 }  // namespace mf
 
 using namespace mf;
@@ -80,7 +81,10 @@ struct Context {
     hipEvent_t la = nullptr, lb = nullptr;     // one launch
     DevBuf items, spill_a, spill_b, acc, counters, per_count, per_nodes, per_map, tables;
     unsigned long long *host_acc = nullptr;   // pinned
-    uint32_t *host_counters = nullptr;        // pinned: [claim, spill_count]
+    uint32_t *host_counters = nullptr;        // pinned: [claim, count of spill_a, count of spill_b]
+    // Captured re-split rounds, reused while the launch configuration holds.
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint8_t> graph_key;
 };
 
 Context g;
@@ -97,9 +101,12 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+constexpr uint32_t kGraphRounds = 8;   // rounds per captured graph (even: replays keep buffer parity)
+
 struct Tuned {
     int schedule;
     int core;
+    bool graph;
     uint32_t block;
     uint32_t waves_per_cu;
     uint32_t budget;
@@ -122,6 +129,7 @@ Tuned tune(const mf_solve_options *opt) {
     t.tail = (opt && opt->tail > 0) ? static_cast<uint32_t>(opt->tail) : t.budget;
     t.events = opt ? opt->count_events != 0 : true;
     t.capacity = (opt && opt->queue_capacity > 0) ? opt->queue_capacity : 0;
+    t.graph = !(opt && opt->graph == 2);
     return t;
 }
 
@@ -217,35 +225,112 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
             if ((rc = g.spill_a.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
             if ((rc = g.spill_b.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
             WorkItem *bufs[2] = {static_cast<WorkItem *>(g.spill_a.ptr), static_cast<WorkItem *>(g.spill_b.ptr)};
+            uint32_t *counts = claim + 1;   // counts[q]: items written to bufs[q] by the last round into it
             const WorkItem *in = dev_items;
             uint64_t in_num = num;
             int next = 0;
+            auto check_errors = [&]() -> int {
+                if (g.host_acc[ACC_ERROR] & 6ull)
+                    return fail(MF_ERR_INTERNAL, "device consistency check failed (%llu)", g.host_acc[ACC_ERROR]);
+                if (g.host_acc[ACC_ERROR] != 0)
+                    return fail(MF_ERR_CAPACITY, "spill queue overflow (capacity %llu items)",
+                                static_cast<unsigned long long>(cap));
+                return MF_OK;
+            };
             HIP_TRY(hipEventRecord(g.ev0, g.stream));
-            while (in_num > 0) {
+            // Host-driven rounds: the first one always (its input is the
+            // frontier), every one when graphs are off.
+            while (in_num > 0 && (!t.graph || rounds == 0)) {
                 float one = 0.0f;
                 if (rounds >= 100000) return fail(MF_ERR_INTERNAL, "re-split did not converge");
-                HIP_TRY(hipMemsetAsync(claim, 0, 2 * sizeof(uint32_t), g.stream));
+                HIP_TRY(hipMemsetAsync(claim, 0, sizeof(uint32_t), g.stream));
+                HIP_TRY(hipMemsetAsync(counts + next, 0, sizeof(uint32_t), g.stream));
                 args.items = in;
                 args.num_items = static_cast<uint32_t>(in_num);
+                args.num_items_dev = nullptr;
                 args.spill = bufs[next];
+                args.spill_count = counts + next;
                 args.spill_cap = static_cast<uint32_t>(cap);
                 HIP_TRY(hipEventRecord(g.la, g.stream));
                 HIP_TRY(launch(args, true, grid));
                 HIP_TRY(hipEventRecord(g.lb, g.stream));
-                HIP_TRY(hipMemcpyAsync(g.host_counters, claim, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+                HIP_TRY(hipMemcpyAsync(g.host_counters, counts + next, sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
                 HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, args.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost, g.stream));
                 HIP_TRY(hipStreamSynchronize(g.stream));
                 HIP_TRY(hipEventElapsedTime(&one, g.la, g.lb));
                 launch_ms += one;
                 ++rounds;
-                if (g.host_acc[ACC_ERROR] & 6ull) return fail(MF_ERR_INTERNAL, "device consistency check failed (%llu)",
-                                                               g.host_acc[ACC_ERROR]);
-                if (g.host_acc[ACC_ERROR] != 0)
-                    return fail(MF_ERR_CAPACITY, "spill queue overflow (capacity %llu items)",
-                                static_cast<unsigned long long>(cap));
+                if ((rc = check_errors()) != MF_OK) return rc;
                 in = bufs[next];
-                in_num = g.host_counters[1];
+                in_num = g.host_counters[0];
                 next ^= 1;
+            }
+            if (in_num > 0) {
+                // Graph rounds: kGraphRounds launches captured once, each
+                // preceded by a reset kernel and reading its item count from
+                // the previous round's spill counter; replayed until the last
+                // round of a replay spills nothing (trailing empty rounds exit
+                // at once).  kGraphRounds is even, so every replay starts on
+                // the same buffer.
+                const int p0 = (in == bufs[0]) ? 0 : 1;
+                Args ga[2];
+                for (int q = 0; q < 2; ++q) {
+                    ga[q] = args;
+                    ga[q].items = bufs[q];
+                    ga[q].num_items = 0;
+                    ga[q].num_items_dev = counts + q;
+                    ga[q].spill = bufs[q ^ 1];
+                    ga[q].spill_count = counts + (q ^ 1);
+                    ga[q].spill_cap = static_cast<uint32_t>(cap);
+                }
+                std::vector<uint8_t> key(2 * sizeof(Args) + sizeof(void *) + 4 * sizeof(uint32_t));
+                uint8_t *kp = key.data();
+                std::memcpy(kp, &ga[0], sizeof(Args));
+                std::memcpy(kp + sizeof(Args), &ga[1], sizeof(Args));
+                const uint32_t cfg[4] = {static_cast<uint32_t>(p0), grid, t.block, lds_bytes};
+                std::memcpy(kp + 2 * sizeof(Args), &kpersist, sizeof(void *));
+                std::memcpy(kp + 2 * sizeof(Args) + sizeof(void *), cfg, sizeof(cfg));
+                if (g.graph == nullptr || key != g.graph_key) {
+                    if (g.graph) (void)hipGraphExecDestroy(g.graph);
+                    g.graph = nullptr;
+                    g.graph_key.clear();
+                    hipGraph_t graph = nullptr;
+                    HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+                    for (uint32_t k = 0; k < kGraphRounds; ++k) {
+                        const int q = p0 ^ static_cast<int>(k & 1u);
+                        hipError_t e = launch_round_reset(claim, counts + (q ^ 1), g.stream);
+                        if (e == hipSuccess) e = launch(ga[q], true, grid);
+                        if (e != hipSuccess) {
+                            (void)hipStreamEndCapture(g.stream, &graph);
+                            if (graph) (void)hipGraphDestroy(graph);
+                            return fail(MF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+                        }
+                    }
+                    HIP_TRY(hipStreamEndCapture(g.stream, &graph));
+                    hipGraphExec_t exec = nullptr;
+                    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    if (e != hipSuccess) return fail(MF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+                    g.graph = exec;
+                    g.graph_key = key;
+                }
+                while (in_num > 0) {
+                    float one = 0.0f;
+                    if (rounds >= 100000) return fail(MF_ERR_INTERNAL, "re-split did not converge");
+                    HIP_TRY(hipEventRecord(g.la, g.stream));
+                    HIP_TRY(hipGraphLaunch(g.graph, g.stream));
+                    HIP_TRY(hipEventRecord(g.lb, g.stream));
+                    HIP_TRY(hipMemcpyAsync(g.host_counters, counts + p0, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                           g.stream));
+                    HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, args.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost,
+                                           g.stream));
+                    HIP_TRY(hipStreamSynchronize(g.stream));
+                    HIP_TRY(hipEventElapsedTime(&one, g.la, g.lb));
+                    launch_ms += one;
+                    rounds += kGraphRounds;
+                    if ((rc = check_errors()) != MF_OK) return rc;
+                    in_num = g.host_counters[0];
+                }
             }
             HIP_TRY(hipEventRecord(g.ev1, g.stream));
         }
@@ -260,7 +345,8 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
         return fail(MF_ERR_INTERNAL, "device consistency check failed (%llu)", g.host_acc[ACC_ERROR]);
     if (g.host_acc[ACC_ERROR] != 0) return fail(MF_ERR_CAPACITY, "work queue overflow");
     fill_result(out, g.host_acc);
-    out->rounds = rounds;
+    // Persistent launches that had input (graph replays also run empty rounds).
+    out->rounds = (t.schedule == MF_SCHED_STATIC) ? rounds : static_cast<uint32_t>(g.host_acc[ACC_ROUNDS]);
     if (num > 0) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1));
@@ -283,7 +369,8 @@ struct mf_frontier {
     std::vector<LunnonProblem> maps;
     std::vector<int> depths;
     DevBuf items;
-    uint64_t num = 0;
+    uint64_t num = 0;                   // work items
+    uint64_t origins = 0;               // entries of the per-item result arrays (0 = num)
     std::vector<WorkItem> host_items;   // kept for learned reordering
 };
 
@@ -345,10 +432,14 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
     const WorkItem *dev_items = static_cast<const WorkItem *>(f->items.ptr);
     DevBuf reordered;
     if (opt && opt->order && f->num > 0) {
+        // Must be a permutation: a repeated item would be counted twice.
         std::vector<WorkItem> tmp(f->num);
+        std::vector<uint8_t> seen(f->num, 0);
         for (uint64_t k = 0; k < f->num; ++k) {
             const uint32_t src = opt->order[k];
-            if (src >= f->num) return fail(MF_ERR_ARG, "order[%llu] out of range", static_cast<unsigned long long>(k));
+            if (src >= f->num || seen[src])
+                return fail(MF_ERR_ARG, "order is not a permutation (entry %llu)", static_cast<unsigned long long>(k));
+            seen[src] = 1;
             tmp[k] = f->host_items[src];
         }
         if ((rc = reordered.ensure(f->num * sizeof(WorkItem))) != MF_OK) return rc;
@@ -364,7 +455,8 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
             return launch_sym(args, t.core, persistent, grid, t.block, lds, t.events, g.stream);
         };
         rc = run_solve(a, sym_kernel_ptr(t.core, false, t.events), sym_kernel_ptr(t.core, true, t.events), lds, t,
-                       dev_items, f->num, f->num, per_count, per_nodes, nullptr, 0, out, launch);
+                       dev_items, f->num, f->origins ? f->origins : f->num, per_count, per_nodes, nullptr, 0, out,
+                       launch);
     } else {
         LunnonArgs a;
         std::vector<uint8_t> blob;
@@ -384,7 +476,8 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         };
         rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events), lunnon_kernel_ptr(t.core, true, t.events), lds,
                        t, dev_items,
-                       f->num, f->num, per_count, per_nodes, per_map, static_cast<uint32_t>(f->maps.size()), out,
+                       f->num, f->origins ? f->origins : f->num, per_count, per_nodes, per_map,
+                       static_cast<uint32_t>(f->maps.size()), out,
                        launch);
     }
     reordered.release();
@@ -493,6 +586,9 @@ void mf_shutdown(void) {
     if (g.device < 0) return;
     (void)hipSetDevice(g.device);
     if (g.stream) (void)hipStreamSynchronize(g.stream);
+    if (g.graph) (void)hipGraphExecDestroy(g.graph);
+    g.graph = nullptr;
+    g.graph_key.clear();
     for (DevBuf *b : {&g.items, &g.spill_a, &g.spill_b, &g.acc, &g.counters, &g.per_count, &g.per_nodes, &g.per_map,
                       &g.tables})
         b->release();
@@ -736,6 +832,78 @@ int mf_sym_solve(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t
     rc = mf_frontier_solve(f, opt, per_count, per_nodes, out);
     mf_frontier_destroy(f);
     return rc;
+}
+
+int mf_sym_expand(int32_t n, int32_t level, const uint8_t *cycles, int64_t count, uint8_t *out_cycles,
+                  uint32_t *out_parent, int64_t max_out, int64_t *written) {
+    if (!written || count < 0 || (count > 0 && !cycles) || max_out < 0 || (max_out > 0 && (!out_cycles || !out_parent)))
+        return fail(MF_ERR_ARG, "bad expand arguments");
+    if (count > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many cases");
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (level < 1 || level > pr.n2 - 2) return fail(MF_ERR_DEPTH, "level must be between 1 and %d", pr.n2 - 2);
+    int64_t w = 0;
+    for (int64_t s = 0; s < count; ++s) {
+        const uint8_t *src = cycles + s * 64;
+        if (!valid_cycle(src, level))
+            return fail(MF_ERR_ARG, "case %lld is not one cycle through cells 0..%d", static_cast<long long>(s), level - 1);
+        const uint64_t good = sym_good(pr, src, level);
+        for (uint64_t gm = good; gm; gm &= gm - 1, ++w) {
+            if (w >= max_out) continue;
+            const int j = __builtin_ctzll(gm);
+            uint8_t *dst = out_cycles + w * 64;
+            std::memcpy(dst, src, 64);
+            dst[level] = dst[j];
+            dst[j] = static_cast<uint8_t>(level);
+            out_parent[w] = static_cast<uint32_t>(s);
+        }
+    }
+    *written = w;
+    if (w > max_out) return fail(MF_ERR_CAPACITY, "%lld children, room for %lld", static_cast<long long>(w),
+                                 static_cast<long long>(max_out));
+    return MF_OK;
+}
+
+int mf_sym_pieces_create(int32_t n, const uint8_t *cycles, const uint8_t *levels, const uint32_t *origins,
+                         const uint64_t *mults, int64_t num, int64_t num_origins, mf_frontier **out) {
+    int rc = require_init();
+    if (rc != MF_OK) return rc;
+    if (!out || num < 0 || num_origins < 0 || (num > 0 && (!cycles || !levels || !origins)))
+        return fail(MF_ERR_ARG, "bad pieces arguments");
+    if (num > 0xffffffffll || num_origins > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many pieces");
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 2) return fail(MF_ERR_ARG, "n must be at least 2 for the cycle DFS");
+    int lo = pr.n2;
+    for (int64_t s = 0; s < num; ++s) {
+        const int level = levels[s];
+        const uint64_t m = mults ? mults[s] : 1;
+        if (level < 1 || level > pr.n2 - 1)
+            return fail(MF_ERR_DEPTH, "piece %lld: level %d outside [1, %d]", static_cast<long long>(s), level, pr.n2 - 1);
+        if (origins[s] >= static_cast<uint64_t>(num_origins))
+            return fail(MF_ERR_ARG, "piece %lld: origin out of range", static_cast<long long>(s));
+        if (m == 0 || m > 0xffff) return fail(MF_ERR_ARG, "multiplier out of range at piece %lld", static_cast<long long>(s));
+        if (!valid_cycle(cycles + s * 64, level))
+            return fail(MF_ERR_ARG, "piece %lld is not one cycle through cells 0..%d", static_cast<long long>(s), level - 1);
+        lo = std::min(lo, level);
+    }
+    auto *f = new mf_frontier();
+    f->kind = 1;
+    f->sym = pr;
+    f->level = num > 0 ? lo : pr.n2 - 1;
+    f->stop = pr.n2;
+    f->host_items.reserve(static_cast<size_t>(num));
+    for (int64_t s = 0; s < num; ++s)
+        f->host_items.push_back(sym_item(cycles + s * 64, mults ? mults[s] : 1, levels[s], origins[s]));
+    if ((rc = upload_items(f)) != MF_OK) {
+        delete f;
+        return rc;
+    }
+    f->origins = static_cast<uint64_t>(num_origins);
+    *out = f;
+    return MF_OK;
 }
 
 int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases, const mf_solve_options *opt,

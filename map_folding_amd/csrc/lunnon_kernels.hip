@@ -320,6 +320,18 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
 template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
+    // Graph rounds read their item count from the previous round's spill
+    // counter; an empty round returns at once (uniformly for the grid).
+    const uint32_t num_items = a.num_items_dev ? *a.num_items_dev : a.num_items;
+    if (num_items == 0) return;
+    // A graph round's input is the previous round's spill queue, which has the
+    // same capacity as this round's: a count past it means that round
+    // overflowed (already flagged) -- read nothing.
+    if (a.num_items_dev != nullptr && num_items > a.spill_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.acc + ACC_ERROR, 1ull);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.acc + ACC_ROUNDS, 1ull);
     const uint32_t span = tables_span(a.tables_bytes);
     lstage(a, lds, span);
     const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
@@ -342,13 +354,13 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                 uint32_t got = 0;
                 if (lane == first) got = atomicAdd(a.claim, want);
                 got = __shfl(got, first, 64);
-                if (got + want >= a.num_items) drained = true;
+                if (got + want >= num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
+                    if (idx < num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
                 }
             } else {
-                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.num_items;
+                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= num_items;
             }
         }
         if (!__any(w.have)) break;

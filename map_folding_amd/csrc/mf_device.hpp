@@ -38,6 +38,7 @@ enum AccSlot {
     ACC_ITEMS,         // work items finished or spilled
     ACC_SPILLED,       // items written to the spill queue
     ACC_ERROR,         // nonzero: capacity overflow or consistency failure
+    ACC_ROUNDS,        // persistent launches that had input
     ACC_SLOTS
 };
 
@@ -55,6 +56,7 @@ struct SymArgs {
     uint32_t edge_word[64];     // 4 pages of cell c packed into one word (partner, n2 = none)
     const WorkItem *items;
     uint32_t num_items;
+    const uint32_t *num_items_dev;  // persistent: when set, the item count is read here (graph rounds)
     uint32_t last;              // n2 - 1: the level whose good mask is counted
     uint32_t base_level;        // lowest level any item starts at (stack slot 0)
     uint32_t levels;            // stack slots per lane
@@ -76,6 +78,7 @@ struct LunnonArgs {
     LunnonMap maps[kMaxMaps];
     const WorkItem *items;
     uint32_t num_items;
+    const uint32_t *num_items_dev;  // persistent: when set, the item count is read here (graph rounds)
     uint32_t levels;
     uint32_t budget;
     uint32_t tail;

@@ -331,6 +331,18 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
 template <typename Stack, bool EV>
 __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
+    // Graph rounds read their item count from the previous round's spill
+    // counter; an empty round returns at once (uniformly for the grid).
+    const uint32_t num_items = a.num_items_dev ? *a.num_items_dev : a.num_items;
+    if (num_items == 0) return;
+    // A graph round's input is the previous round's spill queue, which has the
+    // same capacity as this round's: a count past it means that round
+    // overflowed (already flagged) -- read nothing.
+    if (a.num_items_dev != nullptr && num_items > a.spill_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.acc + ACC_ERROR, 1ull);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.acc + ACC_ROUNDS, 1ull);
     stage_tables(a, lds);
     const uint32_t lane = threadIdx.x & 63u;
     const Lane ln = make_lane<Stack>(a.levels, a.base_level);
@@ -352,15 +364,15 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                 uint32_t got = 0;
                 if (lane == first) got = atomicAdd(a.claim, want);
                 got = __shfl(got, first, 64);
-                if (got + want >= a.num_items) drained = true;
+                if (got + want >= num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < a.num_items) load_item<Stack>(lds, ln, a.items + idx, w);
+                    if (idx < num_items) load_item<Stack>(lds, ln, a.items + idx, w);
                 }
             } else {
                 // Every lane is busy: look at the shared cursor anyway, so that
                 // a wave of long items still learns that the input is gone.
-                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.num_items;
+                drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= num_items;
             }
         }
         if (!__any(w.have)) break;
@@ -405,6 +417,20 @@ const void *pick(bool persistent, bool events) {
 // core: 1 = wide u64 stack, otherwise the compact 16-bit stack.
 const void *sym_kernel_ptr(int core, bool persistent, bool events) {
     return core == 1 ? pick<WideStack>(persistent, events) : pick<CompactStack>(persistent, events);
+}
+
+// Between graph rounds: rewind the claim cursor and empty the next spill
+// queue (a kernel, not a memset node, so that replays stay exact).
+__global__ void round_reset_kernel(uint32_t *claim, uint32_t *out_count) {
+    if (threadIdx.x == 0) {
+        *claim = 0u;
+        *out_count = 0u;
+    }
+}
+
+hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s) {
+    hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, s, claim, out_count);
+    return hipGetLastError();
 }
 
 hipError_t launch_sym(const SymArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,

@@ -24,8 +24,11 @@ SCHEDULES = {
     "resplit_wide": dict(schedule=1, core=1),
     "resplit_wide_tight": dict(schedule=1, core=1, budget=4, tail=4),
     "resplit_tiny": dict(schedule=1, budget=2, tail=2),
+    # host-driven rounds instead of the captured HIP graph of rounds
+    "resplit_nograph": dict(schedule=1, graph=2),
+    "resplit_tight_nograph": dict(schedule=1, budget=8, tail=8, graph=2),
 }
-LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight"]
+LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight", "resplit_tight_nograph"]
 
 
 def _opts(mf, name, **kw):
@@ -112,7 +115,7 @@ def _chunk(golden, n, e, start, mx):
 
 
 @pytest.mark.parametrize("sched", ["resplit", "resplit_tight", "resplit_tiny", "resplit_wide",
-                                   "resplit_wide_tight"])
+                                   "resplit_wide_tight", "resplit_nograph", "resplit_tight_nograph"])
 def test_sym_8x8_first_300_events(gpu, golden, sched):
     _say(f"sym 8x8 d36 [0,300) schedule={sched}")
     g = _chunk(golden, 8, 36, 0, 300)
@@ -225,3 +228,41 @@ def test_cli_sym_fold_chunk(gpu, golden):
     assert r.stdout == f"{g['answer']}\n"
     r = _run([os.path.join(BIN, "sym_fold"), "6", "8"])
     assert r.stdout == f"{golden['square_totals']['6']}\n"
+
+
+def test_graph_rounds_match_host_rounds(gpu):
+    """The captured graph of rounds and host-driven rounds count the same, and
+    a cached graph is replayed correctly on repeated solves."""
+    _say("graph vs host rounds, 8x8 d36 [120000000,+2000), tight budget")
+    cyc, mul = gpu.sym_cases(8, 36, 120000000, 2000)
+    f = gpu.SymFrontier(8, 36, cyc, mul)
+    ref, pc0, pn0 = f.solve(_opts(gpu, "resplit_tight_nograph"), per_item=True)
+    for rep in range(3):
+        r, pc, pn = f.solve(_opts(gpu, "resplit_tight"), per_item=True)
+        assert r.count == ref.count and r.nodes == ref.nodes and r.walks == ref.walks and r.steps == ref.steps
+        assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0)
+        assert r.rounds >= 2
+    f.close()
+
+
+def test_pieces_and_orders_keep_counts(gpu):
+    """Mixed-level pieces (split frontiers) add up per origin to the unsplit
+    counts under any schedule and order; a non-permutation order is refused."""
+    from map_folding_amd import cost
+    _say("pieces: 7x7 d24 [1000,+300) split by the lookahead proxy")
+    cyc, mul = gpu.sym_cases(7, 24, 1000, 300)
+    ref, pc0, pn0 = gpu.sym_solve(7, 24, cyc, mul, _opts(gpu, "resplit"), per_item=True)
+    pieces = cost.lookahead_split(7, 24, cyc, mul, 3.0, depth=4, max_extra=3)
+    assert len(pieces) > cyc.shape[0] and len(set(pieces.levels.tolist())) > 1
+    f = pieces.frontier(7)
+    assert f.num == 300 and f.items == len(pieces)
+    rng = np.random.default_rng(0)
+    for sched, order in (("static", None), ("resplit", pieces.order()), ("resplit_tight", rng.permutation(f.items)),
+                         ("static", rng.permutation(f.items))):
+        r, pc, pn = f.solve(_opts(gpu, sched, order=order), per_item=True)
+        assert r.count == ref.count and np.array_equal(pc, pc0), sched
+        assert int(pn.sum()) + pieces.host_nodes == ref.nodes
+    bad = np.zeros(f.items, dtype=np.uint32)
+    with pytest.raises(gpu.MFError):
+        f.solve(_opts(gpu, "resplit", order=bad))
+    f.close()

@@ -116,3 +116,16 @@ def test_cli_argument_contract():
     assert "center_cases=9394" in r.stderr and "reduced_cases=1225" in r.stderr
     r = _run([os.path.join(BIN, "sym_fold"), "8", "1"], {"MF_EXPANDED_N": "64"})
     assert r.returncode == 1
+
+
+def test_sym_expand_matches_oracle(mf, oracle):
+    """Children at level+1 in ascending child order (recurse_generate,
+    sym_fold.cpp:252-265): per-parent counts and the depth-16 prefixes."""
+    import numpy as np
+    pre = oracle.sym_prefixes(5, 12)
+    ch, par = mf.sym_expand(5, 12, pre)
+    assert np.array_equal(np.bincount(par, minlength=len(pre)),
+                          [oracle.sym_count_to(5, c, 12, 13)[0] for c in pre])
+    assert np.array_equal(ch[:, :13], oracle.sym_prefixes(5, 13)[:, :13])
+    with pytest.raises(mf.MFError):
+        mf.sym_expand(5, 24, pre)   # no children past level n^2 - 2
