@@ -89,6 +89,7 @@ def run_lunnon(args, mf, rl):
     batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
                               tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
     events = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core)
+    tuned = None
     t0 = time.time()
     if args.workload == "lunnon6":
         dims, depth = [6, 6], 18
@@ -103,6 +104,20 @@ def run_lunnon(args, mf, rl):
         depths = [rect_depth(p, q) for p, q in shapes]
         want_maps = [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
         want = sum(want_maps)
+        if not args.no_tune:
+            from map_folding_amd import autotune
+            t1 = time.time()
+            cfg = autotune.tune_lunnon_batch(shapes, targets=(256, 1024, 4096, 16384), budgets=(32, 64, 128),
+                                             waves=(16, 32), log=log)
+            depths = cfg["depths"]
+            batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=cfg["budget"],
+                                      waves_per_cu=cfg["waves_per_cu"], core=args.core)
+            tuned = {"depths": depths, "budget": cfg["budget"], "waves_per_cu": cfg["waves_per_cu"],
+                     "kernel_ms": cfg["kernel_ms"], "trials": len(cfg["trials"]), "tune_s": time.time() - t1,
+                     "default_rule_depths": [rect_depth(p, q) for p, q in shapes]}
+            log(f"autotune: depths {min(depths)}..{max(depths)} budget {cfg['budget']} waves "
+                f"{cfg['waves_per_cu']} ({cfg['kernel_ms']:.1f} ms) in {tuned['tune_s']:.1f}s")
+        t0 = time.time()
         batch = mf.LunnonBatchFrontier(shapes, depths)
 
         def solve(opt):
@@ -149,11 +164,17 @@ def run_lunnon(args, mf, rl):
             _, _, _, cst = oracle.lunnon_resume([6, 6], 18, states[:m])
             sample = f"first {m} frontier states, oracle C restatement of mf.c / solve_from_state, 1 thread"
         else:
-            small = [(p, q) for p, q in shapes if p * q <= 20][:8]
-            cst = {"inserts": 0}
-            for p, q in small:
+            # Maps of the draw in order (those of at most 24 leaves, so that no
+            # single map runs for minutes), until about 10 s of CPU work.
+            small, cst = [], {"inserts": 0}
+            for p, q in shapes:
+                if p * q > 24:
+                    continue
                 cst["inserts"] += oracle.lunnon_count([p, q])[1]["inserts"]
-            sample = f"full counts of {small} (oracle C restatement of mf.c), 1 thread"
+                small.append((p, q))
+                if time.perf_counter() - t1 > 10.0:
+                    break
+            sample = f"full counts of the draw's maps {small} (oracle C restatement of mf.c), 1 thread"
         cs = time.perf_counter() - t1
         cpu = {"value": cst["inserts"] / cs, "unit": "inserts/s", "cores": 1, "kind": "port",
                "sample": f"{sample}, {cs:.1f} s"}
@@ -164,12 +185,11 @@ def run_lunnon(args, mf, rl):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic: deterministic Lunnon frontiers",
         "config": {"workload": label, "items": items, "total": str(res.count), "bit_exact_vs_golden": True},
+        "autotune": tuned,
         "items_per_s": items / (ms / 1e3), "nodes": res.nodes, "gapcomps": res.walks, "walk_steps": res.steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s", "frac": achieved / peak,
                      "traffic": None, "ops_per_step": ops,
                      "ops_model": "10*gapcomps + 8*walk_steps + 6*candidates + 8*inserts + 2*completions"},
-        "no_model": no_model,
-        "learned": learned,
         "plain": plain, "cpu_baseline": cpu, "frontier_gen_s": gen_s,
     }
 
@@ -200,6 +220,8 @@ def main():
                     help="sym: load cost-model parameters (.npy) instead of training them on the GPU")
     ap.add_argument("--split-threshold", type=float, default=15.0,
                     help="sym: split cases whose predicted log2(1+nodes) exceeds this")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (re-split budget/waves; rect64: per-map depths too)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -287,6 +309,18 @@ def main():
         log(f"cost model: {train_s:.1f}s to train, rmse {learned['rmse_log2_nodes']:.2f} (log2 nodes), "
             f"spearman {learned['spearman']:.3f} (lookahead-8 proxy {learned['spearman_lookahead8_proxy']:.3f}); "
             f"{len(pieces)} pieces in {split_s:.2f}s")
+
+    tuned = None
+    if not args.no_tune and not (args.budget or args.tail or args.waves_per_cu):
+        from map_folding_amd import autotune
+        t1 = time.time()
+        cfg = autotune.tune_resplit(timed, budgets=(32, 64, 128), tail_mult=(1,), waves=(20, 32), log=log)
+        timed_opts = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=cfg["budget"],
+                                     tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"], core=args.core)
+        tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "kernel_ms")}
+        tuned.update({"trials": len(cfg["trials"]), "tune_s": time.time() - t1})
+        log(f"autotune: budget {cfg['budget']} tail {cfg['tail']} waves {cfg['waves_per_cu']} "
+            f"({cfg['kernel_ms']:.1f} ms) in {tuned['tune_s']:.1f}s")
 
     for w in range(args.warmup):
         r, _, _ = timed.solve(timed_opts)
@@ -395,6 +429,7 @@ def main():
                      "peak_model": "int32 VALU: CUs x 64 lanes x 2.4 GHz"},
         "no_model": no_model,
         "learned": learned,
+        "autotune": tuned,
         "plain": plain,
         "speedup_vs_plain": plain["speedup_batched_vs_plain"] if plain else None,
         "cpu_baseline": cpu,

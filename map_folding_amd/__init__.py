@@ -29,7 +29,7 @@ __all__ = [
     "MF_SCHED_STATIC", "MF_SCHED_RESPLIT", "MFError", "MFResult", "SolveOptions", "init",
     "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
     "sym_cases", "sym_case_total", "sym_solve", "sym_fold", "SymFrontier", "LunnonFrontier",
-    "LunnonBatchFrontier", "SymPieces", "sym_expand",
+    "LunnonBatchFrontier", "SymPieces", "sym_expand", "lunnon_frontier_size",
     "declared_symbols",
 ]
 
@@ -105,6 +105,14 @@ def lunnon_frontier(dims, depth: int) -> np.ndarray:
     check(lib().mf_lunnon_frontier(dp, len(d), depth, _p(out, C.c_uint8), num.value, C.byref(num)),
           "mf_lunnon_frontier")
     return out
+
+
+def lunnon_frontier_size(dims, depth: int) -> int:
+    """Number of canonical frontier states at partition depth D (nothing stored)."""
+    d, dp = _dims(dims)
+    num = C.c_int64()
+    check(lib().mf_lunnon_frontier(dp, len(d), depth, None, 0, C.byref(num)), "mf_lunnon_frontier")
+    return int(num.value)
 
 
 def lunnon_solve(dims, depth: int, states: np.ndarray, options: SolveOptions | None = None,

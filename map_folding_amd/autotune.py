@@ -1,0 +1,140 @@
+"""Autotuner for the scheduling knobs of the GPU DFS.
+
+Every knob here only changes how work is cut and scheduled -- the Lunnon
+partition depth per map, the re-split budget and tail, the resident waves per
+CU, the learned split threshold -- never a count, so the tuner picks the
+fastest configuration by measuring real solves on the GPU (best of `repeats`
+kernel times) and checks every trial's result against the first one.
+
+The reference fixes these by hand (`partition_depth` on the command line of
+`gpu_folding.cu:209-225`, `MF_EXPANDED_N` of `sym_fold.cpp:655-660`); its
+lanes-per-subtree is one, as here (one lane owns one subtree: the walks of a
+node are serial pointer chases that a second lane could not shorten).
+
+Results are plain dicts, so a tuned configuration can be saved as JSON and
+replayed without searching (`save` / `load`).
+"""
+from __future__ import annotations
+
+import json
+import time
+
+from . import LunnonBatchFrontier, SolveOptions, lunnon_frontier_size
+
+__all__ = ["tune_lunnon_batch", "tune_resplit", "depths_for_target", "save", "load"]
+
+
+def _best(fn, repeats):
+    best = None
+    for _ in range(repeats):
+        r = fn()
+        best = r if best is None or r[0] < best[0] else best
+    return best
+
+
+def depths_for_target(shapes, target: int, max_states: int = 2_000_000):
+    """Per map: the shallowest depth whose frontier has at least `target`
+    states (at most `max_states`, and at most p*q - 2)."""
+    out = []
+    for p, q in shapes:
+        n = p * q
+        best = 2
+        for d in range(2, n - 1):
+            k = lunnon_frontier_size([p, q], d)
+            if k > max_states:
+                break
+            best = d
+            if k >= target:
+                break
+        out.append(best)
+    return out
+
+
+def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 256), waves=(16, 32),
+                      repeats=2, log=None):
+    """Pick per-map frontier depths (by a target states-per-map), then the
+    re-split budget and resident waves, for one batched launch of `shapes`.
+
+    Returns {"depths", "budget", "waves_per_cu", "kernel_ms", "trials"}."""
+    say = log or (lambda m: None)
+    trials = []
+    ref_totals = None
+
+    def solve_with(depths, budget, wv):
+        nonlocal ref_totals
+        t0 = time.time()
+        b = LunnonBatchFrontier(shapes, depths)
+        gen_s = time.time() - t0
+        opt = SolveOptions(count_events=False, budget=budget, waves_per_cu=wv)
+
+        def once():
+            totals, r = b.solve_maps(opt)
+            return r.kernel_ms, totals
+        ms, totals = _best(once, repeats)
+        b.close()
+        if ref_totals is None:
+            ref_totals = totals
+        elif totals != ref_totals:
+            raise RuntimeError("a tuning trial changed a map total")
+        trials.append({"depths": list(depths), "budget": budget, "waves_per_cu": wv, "kernel_ms": ms,
+                       "gen_s": gen_s})
+        say(f"  depths {min(depths)}..{max(depths)} budget {budget} waves {wv}: {ms:.1f} ms (gen {gen_s:.2f}s)")
+        return ms
+
+    best = None
+    for t in targets:
+        d = depths_for_target(shapes, t)
+        ms = solve_with(d, budgets[len(budgets) // 2], waves[-1])
+        if best is None or ms < best[0]:
+            best = (ms, d)
+    depths = best[1]
+    best_cfg = (best[0], budgets[len(budgets) // 2], waves[-1])
+    for bu in budgets:
+        for wv in waves:
+            if (bu, wv) == best_cfg[1:]:
+                continue
+            ms = solve_with(depths, bu, wv)
+            if ms < best_cfg[0]:
+                best_cfg = (ms, bu, wv)
+    return {"depths": depths, "budget": best_cfg[1], "waves_per_cu": best_cfg[2], "kernel_ms": best_cfg[0],
+            "trials": trials}
+
+
+def tune_resplit(frontier, budgets=(32, 64, 128), tail_mult=(1, 4), waves=(20, 32), repeats=1, log=None):
+    """Budget, tail (as a multiple of the budget) and resident waves of the
+    persistent re-split schedule on a resident frontier.
+
+    Returns {"budget", "tail", "waves_per_cu", "kernel_ms", "trials"}."""
+    say = log or (lambda m: None)
+    trials = []
+    ref = None
+    best = None
+    for bu in budgets:
+        for tm in tail_mult:
+            for wv in waves:
+                opt = SolveOptions(count_events=False, budget=bu, tail=bu * tm, waves_per_cu=wv)
+
+                def once():
+                    r, _, _ = frontier.solve(opt)
+                    return r.kernel_ms, r.count
+                ms, count = _best(once, repeats)
+                if ref is None:
+                    ref = count
+                elif count != ref:
+                    raise RuntimeError("a tuning trial changed the count")
+                trials.append({"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "kernel_ms": ms})
+                say(f"  budget {bu} tail {bu * tm} waves {wv}: {ms:.1f} ms")
+                if best is None or ms < best["kernel_ms"]:
+                    best = {"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "kernel_ms": ms}
+    best["trials"] = trials
+    return best
+
+
+def save(cfg: dict, path: str) -> None:
+    with open(path, "w") as f:
+        json.dump(cfg, f, indent=1, sort_keys=True)
+
+
+def load(path: str) -> dict:
+    with open(path) as f:
+        return json.load(f)

@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mapfold.h"
@@ -396,12 +398,23 @@ int lunnon_args(const mf_frontier &f, LunnonArgs &a, std::vector<uint8_t> &blob,
     for (size_t k = 0; k < f.maps.size(); ++k) {
         const LunnonProblem &pr = f.maps[k];
         LunnonMap m;
-        m.table_off = static_cast<uint32_t>(blob.size());
+        // Maps of the same shape share one table (LDS holds every table of
+        // the launch, so duplicates would cost occupancy).
+        size_t same = k;
+        for (size_t q = 0; q < k; ++q)
+            if (f.maps[q].d == pr.d) {
+                same = q;
+                break;
+            }
+        m.table_off = same < k ? a.maps[same].table_off : static_cast<uint32_t>(blob.size());
         m.n = static_cast<uint16_t>(pr.n);
         m.ndim = static_cast<uint8_t>(pr.ndim);
         m.base = static_cast<uint8_t>(f.depths[k] + 1);
         a.maps[k] = m;
-        blob.insert(blob.end(), pr.d.begin(), pr.d.end());
+        if (same == k)   // lower triangle of each dimension: d[i][l][m], m <= l
+            for (int i = 0; i < pr.ndim; ++i)
+                for (int l = 0; l <= pr.n; ++l)
+                    for (int mm = 0; mm <= l; ++mm) blob.push_back(pr.at(i, l, mm));
         lv = std::max<uint32_t>(lv, static_cast<uint32_t>(pr.n - f.depths[k] - 1));
     }
     a.num_maps = static_cast<uint32_t>(f.maps.size());
@@ -688,14 +701,44 @@ int mf_lunnon_batch_frontier_create(const int32_t *dims2, const int32_t *depths,
             delete f;
             return fail(MF_ERR_DEPTH, "map %d: depth must be between 1 and %d", k, pr.n - 1);
         }
-        const int64_t cnt = lunnon_frontier(pr, d, nullptr, 0);
-        std::vector<uint8_t> st(static_cast<size_t>(cnt) * (d + 1));
-        lunnon_frontier(pr, d, st.data(), cnt);
-        for (int64_t s = 0; s < cnt; ++s)
-            f->host_items.push_back(lunnon_item(st.data() + s * (d + 1), d, pr.n,
-                                                static_cast<uint32_t>(f->host_items.size()), k));
         f->maps.push_back(pr);
         f->depths.push_back(d);
+    }
+    // Frontiers of distinct (shape, depth) pairs, collected on host threads;
+    // items are then laid out map by map, in map order.
+    std::vector<int> first(nmaps);
+    for (int k = 0; k < nmaps; ++k) {
+        first[k] = k;
+        for (int q = 0; q < k; ++q)
+            if (f->depths[q] == f->depths[k] && dims2[2 * q] == dims2[2 * k] && dims2[2 * q + 1] == dims2[2 * k + 1]) {
+                first[k] = q;
+                break;
+            }
+    }
+    std::vector<std::vector<uint8_t>> states(nmaps);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k = next++; k < nmaps; k = next++) {
+            if (first[k] != k) continue;
+            const int d = f->depths[k];
+            const int64_t cnt = lunnon_frontier(f->maps[k], d, nullptr, 0);
+            states[k].resize(static_cast<size_t>(cnt) * (d + 1));
+            lunnon_frontier(f->maps[k], d, states[k].data(), cnt);
+        }
+    };
+    const int nthreads = std::max(1, std::min<int>(std::min<int>(nmaps, 16),
+                                                   static_cast<int>(std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    for (int k = 0; k < nmaps; ++k) {
+        const int d = f->depths[k];
+        const std::vector<uint8_t> &st = states[first[k]];
+        const int64_t cnt = static_cast<int64_t>(st.size() / (d + 1));
+        for (int64_t s2 = 0; s2 < cnt; ++s2)
+            f->host_items.push_back(lunnon_item(st.data() + s2 * (d + 1), d, f->maps[k].n,
+                                                static_cast<uint32_t>(f->host_items.size()), k));
     }
     if (f->host_items.size() > 0xffffffffull) {
         delete f;

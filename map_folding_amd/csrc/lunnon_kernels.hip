@@ -17,7 +17,8 @@
 //   * the stack uses the encodings of mf_dfs.hpp (compact by default: a
 //     level whose unrecorded positions are needed recomputes its gaps,
 //     uncounted, above the current one);
-//   * the d tables of every map in the launch sit in LDS as bytes.
+//   * the d tables of every map in the launch sit in LDS as bytes, lower
+//     triangles only (d[i][l][m] for m <= l), one copy per distinct shape.
 // Child order differs from the reference's gap-stack order; subtree counts,
 // inserts and completions do not depend on it.
 #include <hip/hip_runtime.h>
@@ -37,6 +38,7 @@ struct LWork {
     uint32_t n;
     uint32_t ndim;
     uint32_t tab;     // LDS byte address of the map's d table
+    uint32_t tri;     // bytes per dimension of the triangular table: (n+1)(n+2)/2
     uint32_t origin;
     uint32_t map;
     uint32_t floor;   // 0, or j+1: recompute the gaps of leaf l, keep those above j
@@ -71,12 +73,14 @@ __device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m,
 // trips on a state that no folding DFS can produce.
 __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
                                              uint32_t &steps, uint32_t &cands) {
-    const uint32_t l = w.l, s = w.n + 1;
+    const uint32_t l = w.l;
     const uint64_t all = below_mask(l);
     uint64_t meet = all, reach_any = 0;
     bool constrained = false;
     for (uint32_t i = 0; i < w.ndim; ++i) {
-        const uint32_t row = w.tab + (i * s + l) * s;
+        // Row l of dimension i in the triangular table: entries m <= l (the
+        // walk only looks up l itself and links of placed leaves, all < l).
+        const uint32_t row = w.tab + i * w.tri + ((l * (l + 1u)) >> 1);
         uint32_t m = lds[row + l];
         if (m == l) continue;
         constrained = true;
@@ -158,6 +162,7 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkIte
     w.n = mp.n;
     w.ndim = mp.ndim;
     w.tab = mp.table_off;
+    w.tri = ((mp.n + 1u) * (mp.n + 2u)) >> 1;
     ln.base = mp.base;
     w.map = it->map;
     w.origin = it->origin;

@@ -1,0 +1,39 @@
+"""Autotuner: trials never change a count, and the pick is the fastest trial."""
+import pytest
+
+
+def test_depths_for_target(mf):
+    from map_folding_amd import autotune
+    shapes = [(3, 4), (2, 6), (4, 4)]
+    d = autotune.depths_for_target(shapes, 50)
+    for (p, q), depth in zip(shapes, d):
+        assert 2 <= depth <= p * q - 2
+        k = mf.lunnon_frontier_size([p, q], depth)
+        assert k >= 50 or depth == p * q - 2 or mf.lunnon_frontier_size([p, q], depth + 1) > 2_000_000
+        if depth > 2:
+            assert mf.lunnon_frontier_size([p, q], depth - 1) < 50
+
+
+@pytest.mark.gpu
+def test_tune_lunnon_batch_keeps_totals(gpu, golden):
+    from map_folding_amd import autotune
+    shapes = [(3, 4), (2, 6), (4, 4), (3, 5), (2, 8)]
+    cfg = autotune.tune_lunnon_batch(shapes, targets=(16, 128), budgets=(16, 64), waves=(16, 32), repeats=1)
+    assert cfg["kernel_ms"] == min(t["kernel_ms"] for t in cfg["trials"])
+    b = gpu.LunnonBatchFrontier(shapes, cfg["depths"])
+    totals, _ = b.solve_maps(gpu.SolveOptions(budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"]))
+    assert totals == [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
+    b.close()
+
+
+@pytest.mark.gpu
+def test_tune_resplit_keeps_count(gpu, golden):
+    from map_folding_amd import autotune
+    cyc, mul = gpu.sym_cases(7, 24, 200000, 2000)
+    f = gpu.SymFrontier(7, 24, cyc, mul)
+    ref, _, _ = f.solve()
+    cfg = autotune.tune_resplit(f, budgets=(16, 64), tail_mult=(1, 4), waves=(20, 32))
+    assert len(cfg["trials"]) == 8 and cfg["kernel_ms"] == min(t["kernel_ms"] for t in cfg["trials"])
+    r, _, _ = f.solve(gpu.SolveOptions(budget=cfg["budget"], tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"]))
+    assert r.count == ref.count
+    f.close()
