@@ -220,6 +220,11 @@ def main():
                     help="sym: load cost-model parameters (.npy) instead of training them on the GPU")
     ap.add_argument("--split-threshold", type=float, default=15.0,
                     help="sym: split cases whose predicted log2(1+nodes) exceeds this")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="sym: time only the batched+learned schedule (no no-model step, no plain kernel); "
+                         "used for profiling runs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="host-driven re-split rounds instead of the captured HIP graph of rounds")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (re-split budget/waves; rect64: per-map depths too)")
     args = ap.parse_args()
@@ -251,11 +256,12 @@ def main():
     log(f"frontier: {cyc.shape[0]} cases from {args.start} generated in {gen_s:.2f}s (host)")
     front = mf.SymFrontier(n, level, cyc, mul)
 
+    graph = 2 if args.no_graph else 0
     batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
-                              tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
+                              tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core, graph=graph)
     # Exact event counts (schedule-independent) for the algorithmic-work figure.
     res, _, case_nodes = front.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True,
-                                                     core=args.core), per_item=True)
+                                                     core=args.core, graph=graph), per_item=True)
     nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
     total = res.count * n * n
     golden = golden0 if (args.start, args.cases) == (start0, cases0) else None
@@ -265,16 +271,19 @@ def main():
         f"kernel_ms={res.kernel_ms:.1f}")
 
     # The persistent schedule without the learned part (one step, for reference).
-    r0, _, _ = front.solve(batched)
-    no_model = {"ms": r0.total_ms, "kernel_ms": r0.kernel_ms, "rounds": r0.rounds,
-                "nodes_per_s": nodes / (r0.total_ms / 1e3)}
-    log(f"persistent, no model: {r0.total_ms:.1f} ms (kernels {r0.kernel_ms:.1f} ms, {r0.rounds} rounds)")
+    no_model = None
+    if not args.no_compare:
+        r0, _, _ = front.solve(batched)
+        no_model = {"ms": r0.total_ms, "kernel_ms": r0.kernel_ms, "rounds": r0.rounds,
+                    "nodes_per_s": nodes / (r0.total_ms / 1e3)}
+        log(f"persistent, no model: {r0.total_ms:.1f} ms (kernels {r0.kernel_ms:.1f} ms, {r0.rounds} rounds)")
 
     # Learned part: train the cost model on the GPU (pairs logged by the GPU DFS
     # on windows disjoint from this chunk), predict every case, split the ones
     # predicted large into subtrees, order the pieces largest-first.
     learned = None
     timed, timed_opts, gpu_nodes = front, batched, nodes
+    n_inputs = cyc.shape[0]
     if not args.no_learned:
         from map_folding_amd import cost
         t1 = time.time()
@@ -293,13 +302,16 @@ def main():
         split_s = time.time() - t1
         timed = pieces.frontier(n)
         timed_opts = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
-                                     tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
+                                     tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core,
+                                     graph=2 if args.no_graph else 0)
         rl_ev, pc_l, _ = timed.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True,
-                                                     core=args.core), per_item=True)
+                                                     core=args.core, graph=2 if args.no_graph else 0),
+                                     per_item=True)
         if rl_ev.count * n * n != total or rl_ev.nodes + pieces.host_nodes != nodes:
             raise SystemExit("learned pieces disagree with the case frontier")
         gpu_nodes = rl_ev.nodes
         walks, steps = rl_ev.walks, rl_ev.steps
+        n_inputs = len(pieces)
         learned = {"pairs": pairs, "train_s": train_s, "final_loss": float(losses[-1]) if len(losses) else None,
                    "rmse_log2_nodes": float(np.sqrt(np.mean((pred - y) ** 2))),
                    "spearman": cost.spearman(pred, case_nodes),
@@ -316,7 +328,8 @@ def main():
         t1 = time.time()
         cfg = autotune.tune_resplit(timed, budgets=(32, 64, 128), tail_mult=(1,), waves=(20, 32), log=log)
         timed_opts = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=cfg["budget"],
-                                     tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"], core=args.core)
+                                     tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"], core=args.core,
+                                     graph=graph)
         tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "kernel_ms")}
         tuned.update({"trials": len(cfg["trials"]), "tune_s": time.time() - t1})
         log(f"autotune: budget {cfg['budget']} tail {cfg['tail']} waves {cfg['waves_per_cu']} "
@@ -326,7 +339,7 @@ def main():
         r, _, _ = timed.solve(timed_opts)
         log(f"warmup {w}: {r.total_ms:.1f} ms")
 
-    wall, kern, span, rounds = [], [], [], []
+    wall, kern, span, rounds, items = [], [], [], [], []
     for k in range(args.steps):
         t1 = time.perf_counter()
         r, _, _ = timed.solve(timed_opts)
@@ -336,7 +349,9 @@ def main():
         kern.append(r.kernel_ms / 1e3)
         span.append(r.span_ms / 1e3)
         rounds.append(r.rounds)
-        log(f"step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}")
+        items.append(r.items)
+        log(f"step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}, "
+            f"items {r.items}")
 
     ms_per_step = 1e3 * sum(wall) / len(wall)
     node_rate = gpu_nodes / (ms_per_step / 1e3)
@@ -351,7 +366,7 @@ def main():
     # minutes, so both schedules are timed on the chunk's first --plain-cases
     # cases, resident the same way.
     plain = None
-    if not args.no_plain:
+    if not args.no_plain and not args.no_compare:
         m = min(args.plain_cases, cyc.shape[0])
         sub = mf.SymFrontier(n, level, cyc[:m], mul[:m])
         rs, _, _ = sub.solve(mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core))
@@ -422,6 +437,10 @@ def main():
         "nodes": nodes, "page_walks": walks, "walk_steps": steps, "leaves": leaves,
         "rounds": rounds[-1] if rounds else None,
         "kernel_ms_per_step": kernel_s * 1e3,
+        # Every work item (80 B) is read once; every item but the inputs was
+        # written once by the round that split it.
+        "items_per_step": items[-1] if items else None,
+        "algorithmic_bytes_per_launch": (80 * (2 * items[-1] - n_inputs) / rounds[-1]) if items else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": _traffic(args),
                      "ops_per_step": ops,

@@ -336,3 +336,36 @@ def train_default(seed: int = 0, epochs: int = 12, batch: int = 512, lr: float =
     m = CostModel(seed=seed)
     losses = m.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return m, losses, len(y)
+
+
+def main(argv=None) -> int:
+    """python -m map_folding_amd.cost train --out PARAMS.npy [--seed S] [--epochs E]
+
+    Collects the default training pairs with the GPU DFS, trains the model on
+    the GPU and saves its FP32 parameters (np.save; load with CostModel.load
+    or bench.py --cost-params)."""
+    import argparse
+    import json
+    import time
+    from . import init
+    ap = argparse.ArgumentParser(prog="python -m map_folding_amd.cost")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", required=True)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--epochs", type=int, default=12)
+    a = ap.parse_args(argv)
+    init(0)
+    t0 = time.time()
+    data = default_training_set()
+    t1 = time.time()
+    m, losses, pairs = train_default(seed=a.seed, epochs=a.epochs, data=data)
+    m.save(a.out)
+    print(json.dumps({"out": a.out, "pairs": pairs, "collect_s": t1 - t0, "train_s": time.time() - t1,
+                      "epoch_loss": [float(x) for x in losses]}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())
