@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmapfold.so")
+LIB_PATH = os.environ.get("MAPFOLD_LIB") or os.path.join(_HERE, "libmapfold.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "mapfold.h")
 HEADER_PATHS = [HEADER_PATH, os.path.join(os.path.dirname(_HERE), "include", "mapfold_cost.h")]
 

@@ -48,6 +48,7 @@ struct LWork {
     uint32_t isteps;
     uint32_t icands;
     bool have;
+    bool popping;     // backtracking continues this step (no gap computation)
     bool progressed;  // a counted gap computation of this item has completed
     bool bad;         // a walk exceeded l steps: the state was not a folding state
 };
@@ -102,37 +103,49 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
     return constrained ? meet : all;
 }
 
+// Backtracking pops at most this many levels per step (as in the symmetry
+// DFS): a lane with more to pop skips the next step's gap computation.
+#ifndef MF_LUNNON_MAX_POPS
+#define MF_LUNNON_MAX_POPS 4
+#endif
+constexpr uint32_t kLMaxPops = MF_LUNNON_MAX_POPS;
+
 template <typename Stack, bool EV>
 __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w) {
-    uint32_t ns = 0, nc = 0;
-    uint64_t g = gap_mask(lds, ln, w, ns, nc);
-    if (w.bad) return true;
-    if (w.floor == 0) {
-        w.progressed = true;
-        w.inodes += static_cast<uint32_t>(__popcll(g));
-        if (EV) {
-            ++w.iwalks;
-            w.isteps += ns;
-            w.icands += nc;
+    if (!w.popping) {
+        uint32_t ns = 0, nc = 0;
+        uint64_t g = gap_mask(lds, ln, w, ns, nc);
+        if (w.bad) return true;
+        if (w.floor == 0) {
+            w.progressed = true;
+            w.inodes += static_cast<uint32_t>(__popcll(g));
+            if (EV) {
+                ++w.iwalks;
+                w.isteps += ns;
+                w.icands += nc;
+            }
+            g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
+            if (w.l == w.n) {
+                w.comps += static_cast<uint32_t>(__popcll(g));
+                g = 0;
+            }
+        } else {
+            g &= ~below_mask(w.floor);
+            w.floor = 0;
         }
-        g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
-        if (w.l == w.n) {
-            w.comps += static_cast<uint32_t>(__popcll(g));
-            g = 0;
+        if (g != 0) {
+            const typename Stack::Entry e = Stack::encode(g);
+            Stack::put(lds, ln, w.l, e);
+            link(lds, ln, Stack::child(e), w.l);
+            ++w.l;
+            return false;
         }
-    } else {
-        g &= ~below_mask(w.floor);
-        w.floor = 0;
     }
-    if (g != 0) {
-        const typename Stack::Entry e = Stack::encode(g);
-        Stack::put(lds, ln, w.l, e);
-        link(lds, ln, Stack::child(e), w.l);
-        ++w.l;
-        return false;
-    }
-    for (;;) {
-        if (w.l == w.root) return true;
+    for (uint32_t k = 0; k < kLMaxPops; ++k) {
+        if (w.l == w.root) {
+            w.popping = false;
+            return true;
+        }
         --w.l;
         const typename Stack::Entry e = Stack::get(lds, ln, w.l);
         const uint32_t j = Stack::child(e);
@@ -143,13 +156,17 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
             Stack::put(lds, ln, w.l, e2);
             link(lds, ln, Stack::child(e2), w.l);
             ++w.l;
+            w.popping = false;
             return false;
         }
         if (more == 2u) {
             w.floor = j + 1;
+            w.popping = false;
             return false;
         }
     }
+    w.popping = true;
+    return false;
 }
 
 // Item kinds as for the symmetry DFS; the floor of a recompute piece travels
@@ -176,6 +193,7 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkIte
     w.icands = 0;
     w.floor = 0;
     w.have = true;
+    w.popping = false;
     w.progressed = false;
     w.bad = false;
     if (mask == 0) {
@@ -255,7 +273,8 @@ __device__ __forceinline__ void lspill_all(const LunnonArgs &a, uint8_t *lds, co
     uint32_t t = w.l;
     uint64_t mask = 0;
     uint32_t floor = w.floor;
-    bool emit = pend;
+    // A lane in the middle of backtracking has no pending gap computation.
+    bool emit = pend && !w.popping;
     while (__any(pend)) {
         uint32_t cnt = 0;
         const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
@@ -312,6 +331,7 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     LTotals tot;
     LWork w;
     w.have = false;
+    w.popping = false;
     if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
     while (w.have) {
         if (ldfs_step<Stack, EV>(lds, ln, w)) {
@@ -347,6 +367,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     LTotals tot;
     LWork w;
     w.have = false;
+    w.popping = false;
     bool drained = false;
     uint32_t tail_left = a.tail;
 

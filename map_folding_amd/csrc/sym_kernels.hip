@@ -76,6 +76,7 @@ struct Work {
     uint32_t iwalks;
     uint32_t isteps;
     bool have;
+    bool popping;      // backtracking continues this step (no node to expand)
 };
 
 struct Totals {
@@ -113,33 +114,47 @@ __device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, u
 
 // One node: expand level w.i (or recompute it above w.floor), then move to
 // the next node of the subtree.  Returns true when the item is finished.
+// Backtracking pops at most this many levels per step; a lane with more to
+// pop skips the next step's walks and continues.  Bounding the pops keeps one
+// lane's long unwind from holding the whole wave (a lane pops 1.0 levels per
+// node on average, the deepest lane of a wave ~9).
+#ifndef MF_SYM_MAX_POPS
+#define MF_SYM_MAX_POPS 4
+#endif
+constexpr uint32_t kMaxPops = MF_SYM_MAX_POPS;
+
 template <typename Stack, bool EV>
 __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t last, Work &w) {
-    uint32_t nw = 0, ns = 0;
-    uint64_t g = good_mask(lds, ln.cl, w.i, nw, ns);
-    if (w.floor == 0) {
-        ++w.inodes;
-        if (EV) {
-            w.iwalks += nw;
-            w.isteps += ns;
+    if (!w.popping) {
+        uint32_t nw = 0, ns = 0;
+        uint64_t g = good_mask(lds, ln.cl, w.i, nw, ns);
+        if (w.floor == 0) {
+            ++w.inodes;
+            if (EV) {
+                w.iwalks += nw;
+                w.isteps += ns;
+            }
+            if (w.i == last) {
+                w.leaves += static_cast<uint32_t>(__builtin_popcountll(g));
+                g = 0;
+            }
+        } else {
+            g &= ~below_mask(w.floor);
+            w.floor = 0;
         }
-        if (w.i == last) {
-            w.leaves += static_cast<uint32_t>(__builtin_popcountll(g));
-            g = 0;
+        if (g != 0) {
+            const typename Stack::Entry e = Stack::encode(g);
+            Stack::put(lds, ln, w.i, e);
+            place(lds, ln, Stack::child(e), w.i);
+            ++w.i;
+            return false;
         }
-    } else {
-        g &= ~below_mask(w.floor);
-        w.floor = 0;
     }
-    if (g != 0) {
-        const typename Stack::Entry e = Stack::encode(g);
-        Stack::put(lds, ln, w.i, e);
-        place(lds, ln, Stack::child(e), w.i);
-        ++w.i;
-        return false;
-    }
-    for (;;) {
-        if (w.i == w.root) return true;
+    for (uint32_t k = 0; k < kMaxPops; ++k) {
+        if (w.i == w.root) {
+            w.popping = false;
+            return true;
+        }
         --w.i;
         const typename Stack::Entry e = Stack::get(lds, ln, w.i);
         const uint32_t j = Stack::child(e);
@@ -150,13 +165,17 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             Stack::put(lds, ln, w.i, e2);
             place(lds, ln, Stack::child(e2), w.i);
             ++w.i;
+            w.popping = false;
             return false;
         }
         if (more == 2u) {
             w.floor = j + 1;
+            w.popping = false;
             return false;
         }
     }
+    w.popping = true;
+    return false;
 }
 
 // ------------------------------------------------------------ item plumbing
@@ -176,6 +195,7 @@ __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, c
     w.isteps = 0;
     w.floor = 0;
     w.have = true;
+    w.popping = false;
 }
 
 template <typename Stack>
@@ -260,7 +280,9 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
     uint32_t t = w.i;
     uint64_t mask = 0;
     uint32_t floor = w.floor;   // the pending expansion may itself be a recompute
-    bool emit = pend;
+    // A lane in the middle of backtracking has no pending node: levels >= i
+    // are finished, so it only continues the unwind, emitting siblings.
+    bool emit = pend && !w.popping;
     while (__any(pend)) {
         uint32_t cnt = 0;
         const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
@@ -317,6 +339,7 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
     Totals tot;
     Work w;
     w.have = false;
+    w.popping = false;
     if (idx < a.num_items) load_item<Stack>(lds, ln, a.items + idx, w);
     while (w.have) {
         if (dfs_step<Stack, EV>(lds, ln, a.last, w)) {
@@ -351,6 +374,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     Totals tot;
     Work w;
     w.have = false;
+    w.popping = false;
     bool drained = false;   // wave-uniform: the input array has been handed out
     uint32_t tail_left = a.tail;
 
