@@ -71,7 +71,7 @@ float uniform01(uint64_t &s) { return static_cast<float>(splitmix(s) >> 40) * (1
 
 // Activations of one batch (rows x 128 per layer) and their gradients.
 struct Work {
-    Buf x, y, h1, h2, h3, pred, g, d1, d2, d3, partial, idx, loss;
+    Buf x, y, h1, h2, h3, pred, g, d1, d2, d3, partial, idx, loss, part;
     int rows = 0;
     hipError_t ensure(int r, int nbatch_loss) {
         hipError_t e;
@@ -83,10 +83,11 @@ struct Work {
             if ((e = b->ensure(R * 4)) != hipSuccess) return e;
         if ((e = partial.ensure((R / 256 + 1) * 4)) != hipSuccess) return e;
         if ((e = loss.ensure(static_cast<size_t>(std::max(nbatch_loss, 1)) * 4)) != hipSuccess) return e;
+        if ((e = part.ensure(static_cast<size_t>(kSplitK) * kHidden * kHidden * 4)) != hipSuccess) return e;
         return hipSuccess;
     }
     void release() {
-        for (Buf *b : {&x, &y, &h1, &h2, &h3, &pred, &g, &d1, &d2, &d3, &partial, &idx, &loss}) b->release();
+        for (Buf *b : {&x, &y, &h1, &h2, &h3, &pred, &g, &d1, &d2, &d3, &partial, &idx, &loss, &part}) b->release();
     }
 };
 
@@ -125,15 +126,15 @@ hipError_t backward(mf_cost_model *m, int rows, float *loss_dev, hipStream_t s) 
     hipError_t e;
     if ((e = cost_mse(rows, w.pred.f(), w.y.f(), w.g.f(), w.partial.f(), loss_dev, s)) != hipSuccess) return e;
     if ((e = cost_head_bwd(rows, w.g.f(), P + kOffW4, w.h3.f(), G + kOffW4, w.d3.f(), s)) != hipSuccess) return e;
-    if ((e = cost_linear_bwd_weight(rows, kHidden, w.d3.f(), w.h2.f(), G + kOffW3, G + kOffB3, s)) != hipSuccess)
+    if ((e = cost_linear_bwd_weight(rows, kHidden, w.d3.f(), w.h2.f(), G + kOffW3, G + kOffB3, w.part.f(), s)) != hipSuccess)
         return e;
     if ((e = cost_linear_bwd_input(rows, kHidden, w.d3.f(), P + kOffW3, w.h2.f(), w.d2.f(), s)) != hipSuccess)
         return e;
-    if ((e = cost_linear_bwd_weight(rows, kHidden, w.d2.f(), w.h1.f(), G + kOffW2, G + kOffB2, s)) != hipSuccess)
+    if ((e = cost_linear_bwd_weight(rows, kHidden, w.d2.f(), w.h1.f(), G + kOffW2, G + kOffB2, w.part.f(), s)) != hipSuccess)
         return e;
     if ((e = cost_linear_bwd_input(rows, kHidden, w.d2.f(), P + kOffW2, w.h1.f(), w.d1.f(), s)) != hipSuccess)
         return e;
-    return cost_linear_bwd_weight(rows, kFeatures, w.d1.f(), w.x.f(), G + kOffW1, G + kOffB1, s);
+    return cost_linear_bwd_weight(rows, kFeatures, w.d1.f(), w.x.f(), G + kOffW1, G + kOffB1, w.part.f(), s);
 }
 
 // ------------------------------------------------------------------ features

@@ -11,9 +11,11 @@
 //                          of the layer input (backward to the input), and
 //                          plain store (weight gradients).  One kernel covers
 //                          the forward product X W^T (TB), the input gradient
-//                          dY W and the weight gradient dY^T X (TA).
+//                          dY W and the weight gradient dY^T X (TA; split
+//                          over the batch rows into partial tiles that
+//                          sum_partials adds in order).
 //   head_fwd / head_bwd    the 128 -> 1 output layer and the MSE gradient.
-//   bias_grad              column sums of dY (deterministic, no atomics).
+//   column_sums            bias gradients (deterministic, no atomics).
 //   adam_step              Adam with bias correction, one thread per weight.
 //
 // Inference (mlp_infer_bf16) runs the whole network per 16 items on one wave
@@ -43,18 +45,23 @@ enum Epilogue : int { kStore = 0, kBiasRelu = 1, kMaskRelu = 2 };
 //   TB = false: B is row-major K x N (ldb);  TB = true: B is row-major N x K.
 // kBiasRelu: C = max(0, C + bias[n]).  kMaskRelu: C = C * (mask[m, n] > 0),
 // mask row-major M x N with ldc (the forward activation of the layer input).
+// Split K: block z covers K rows [z*kchunk, min(K, (z+1)*kchunk)) and writes
+// its partial product to C + z*M*ldc (summed afterwards in z order).
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float *__restrict__ A, int lda,
                                                 const float *__restrict__ B, int ldb, float *__restrict__ C,
                                                 int ldc, const float *__restrict__ bias,
-                                                const float *__restrict__ mask) {
+                                                const float *__restrict__ mask, int kchunk) {
     __shared__ float as[kSlice][kTile + 4];   // as[k][m]
     __shared__ float bs[kSlice][kTile + 4];   // bs[k][n]
     const int tid = threadIdx.x;
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
     const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    C += static_cast<size_t>(blockIdx.z) * M * ldc;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += kSlice) {
+    for (int k0 = kbeg; k0 < kend; k0 += kSlice) {
         // 64 x 16 elements of each operand, 4 per thread.
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -65,14 +72,14 @@ __global__ __launch_bounds__(256) void gemm_f32(int M, int N, int K, const float
             else    { kk = idx % kSlice; mm = idx / kSlice; } // contiguous along k
             const int gm = m0 + mm, gk = k0 + kk;
             float va = 0.f;
-            if (gm < M && gk < K) va = TA ? A[static_cast<size_t>(gk) * lda + gm] : A[static_cast<size_t>(gm) * lda + gk];
+            if (gm < M && gk < kend) va = TA ? A[static_cast<size_t>(gk) * lda + gm] : A[static_cast<size_t>(gm) * lda + gk];
             as[kk][mm] = va;
             int nn, kb;
             if (TB) { kb = idx % kSlice; nn = idx / kSlice; }
             else    { nn = idx % kTile; kb = idx / kTile; }
             const int gn = n0 + nn, gkb = k0 + kb;
             float vb = 0.f;
-            if (gn < N && gkb < K) vb = TB ? B[static_cast<size_t>(gn) * ldb + gkb] : B[static_cast<size_t>(gkb) * ldb + gn];
+            if (gn < N && gkb < kend) vb = TB ? B[static_cast<size_t>(gn) * ldb + gkb] : B[static_cast<size_t>(gkb) * ldb + gn];
             bs[kb][nn] = vb;
         }
         __syncthreads();
@@ -150,19 +157,39 @@ __global__ __launch_bounds__(256) void head_bwd_input(int rows, const float *__r
     dh[idx] = h[idx] > 0.f ? g[b] * w[k] : 0.f;
 }
 
-// Column sums over `rows` of a row-major [rows x cols] matrix, one thread per
-// column, summed in row order (deterministic).  Used for bias gradients and,
-// with x = h and a row weight g, for the head weights.
-__global__ __launch_bounds__(128) void column_sums(int rows, int cols, const float *__restrict__ x,
+// Column sums over `rows` of a row-major [rows x cols] matrix: 32 columns
+// per block, 8 row groups (rows r = g mod 8) summed in row order, then the
+// groups in order (deterministic).  Used for bias gradients and, with a row
+// weight, for the head weights.
+__global__ __launch_bounds__(256) void column_sums(int rows, int cols, const float *__restrict__ x,
                                                    const float *__restrict__ rowscale, float *__restrict__ out) {
-    const int c = blockIdx.x * 128 + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[8][33];
+    const int cl = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const int c = blockIdx.x * 32 + cl;
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) {
-        const float v = x[static_cast<size_t>(r) * cols + c];
-        s = rowscale ? __builtin_fmaf(v, rowscale[r], s) : s + v;
+    if (c < cols)
+        for (int r = grp; r < rows; r += 8) {
+            const float v = x[static_cast<size_t>(r) * cols + c];
+            s = rowscale ? __builtin_fmaf(v, rowscale[r], s) : s + v;
+        }
+    part[grp][cl] = s;
+    __syncthreads();
+    if (grp == 0 && c < cols) {
+        float t = 0.f;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) t += part[g][cl];
+        out[c] = t;
     }
-    out[c] = s;
+}
+
+// out[i] = sum over z of part[z * n + i], in z order.
+__global__ __launch_bounds__(256) void sum_partials(int parts, int n, const float *__restrict__ part,
+                                                    float *__restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int z = 0; z < parts; ++z) s += part[static_cast<size_t>(z) * n + i];
+    out[i] = s;
 }
 
 __global__ __launch_bounds__(256) void sum_into(int n, const float *__restrict__ x, float *__restrict__ out) {
@@ -339,7 +366,7 @@ static inline unsigned cdiv(size_t a, size_t b) { return static_cast<unsigned>((
 hipError_t cost_linear_fwd(int rows, int in, const float *x, const float *w, const float *b, float *y,
                            hipStream_t s) {
     MF_LAUNCH((gemm_f32<false, true, kBiasRelu>), dim3(cdiv(kHidden, kTile), cdiv(rows, kTile)), 256, 0, s, rows,
-              kHidden, in, x, in, w, in, y, kHidden, b, nullptr);
+              kHidden, in, x, in, w, in, y, kHidden, b, nullptr, in);
     return hipGetLastError();
 }
 
@@ -347,16 +374,21 @@ hipError_t cost_linear_fwd(int rows, int in, const float *x, const float *w, con
 hipError_t cost_linear_bwd_input(int rows, int in, const float *dy, const float *w, const float *x, float *dx,
                                  hipStream_t s) {
     MF_LAUNCH((gemm_f32<false, false, kMaskRelu>), dim3(cdiv(in, kTile), cdiv(rows, kTile)), 256, 0, s, rows, in,
-              kHidden, dy, kHidden, w, in, dx, in, nullptr, x);
+              kHidden, dy, kHidden, w, in, dx, in, nullptr, x, kHidden);
     return hipGetLastError();
 }
 
-// dW = dY^T X (128 x in), db = column sums of dY.
+// dW = dY^T X (128 x in), db = column sums of dY.  The product is split over
+// the batch rows into up to kSplitK partial tiles (in `part`, kSplitK x 128 x
+// in floats) that are then summed in order.
 hipError_t cost_linear_bwd_weight(int rows, int in, const float *dy, const float *x, float *dw, float *db,
-                                  hipStream_t s) {
-    MF_LAUNCH((gemm_f32<true, false, kStore>), dim3(cdiv(in, kTile), cdiv(kHidden, kTile)), 256, 0, s, kHidden, in,
-              rows, dy, kHidden, x, in, dw, in, nullptr, nullptr);
-    MF_LAUNCH(column_sums, cdiv(kHidden, 128), 128, 0, s, rows, kHidden, dy, nullptr, db);
+                                  float *part, hipStream_t s) {
+    const int chunk = std::max(kSlice, ((rows + kSplitK - 1) / kSplitK + kSlice - 1) / kSlice * kSlice);
+    const int parts = (rows + chunk - 1) / chunk;
+    MF_LAUNCH((gemm_f32<true, false, kStore>), dim3(cdiv(in, kTile), cdiv(kHidden, kTile), parts), 256, 0, s,
+              kHidden, in, rows, dy, kHidden, x, in, part, in, nullptr, nullptr, chunk);
+    MF_LAUNCH(sum_partials, cdiv(static_cast<size_t>(kHidden) * in, 256), 256, 0, s, parts, kHidden * in, part, dw);
+    MF_LAUNCH(column_sums, cdiv(kHidden, 32), 256, 0, s, rows, kHidden, dy, nullptr, db);
     return hipGetLastError();
 }
 
@@ -376,8 +408,8 @@ hipError_t cost_mse(int rows, const float *pred, const float *target, float *g, 
 // Head gradients: dw[k] = sum_b g[b] h[b,k], db = sum_b g[b]; dh = g w * (h > 0).
 hipError_t cost_head_bwd(int rows, const float *g, const float *w, const float *h, float *dw, float *dh,
                          hipStream_t s) {
-    MF_LAUNCH(column_sums, cdiv(kHidden, 128), 128, 0, s, rows, kHidden, h, g, dw);
-    MF_LAUNCH(column_sums, 1, 128, 0, s, rows, 1, g, nullptr, dw + kHidden);
+    MF_LAUNCH(column_sums, cdiv(kHidden, 32), 256, 0, s, rows, kHidden, h, g, dw);
+    MF_LAUNCH(column_sums, 1, 256, 0, s, rows, 1, g, nullptr, dw + kHidden);
     MF_LAUNCH(head_bwd_input, cdiv(static_cast<size_t>(rows) * kHidden, 256), 256, 0, s, rows, g, w, h, dh);
     return hipGetLastError();
 }

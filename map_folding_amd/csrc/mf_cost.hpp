@@ -33,8 +33,9 @@ hipError_t cost_linear_fwd(int rows, int in, const float *x, const float *w, con
                            hipStream_t s);
 hipError_t cost_linear_bwd_input(int rows, int in, const float *dy, const float *w, const float *x, float *dx,
                                  hipStream_t s);
+constexpr int kSplitK = 8;   // weight-gradient products: partial tiles over the batch
 hipError_t cost_linear_bwd_weight(int rows, int in, const float *dy, const float *x, float *dw, float *db,
-                                  hipStream_t s);
+                                  float *part, hipStream_t s);
 hipError_t cost_head_fwd(int rows, const float *h, const float *w, float *out, hipStream_t s);
 hipError_t cost_mse(int rows, const float *pred, const float *target, float *g, float *partial, float *loss,
                     hipStream_t s);

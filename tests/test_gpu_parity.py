@@ -3,6 +3,7 @@ golden totals.  Integer results must match bit-exactly, including the exact
 event counters (nodes, walks, steps, leaves) that make up the roofline's
 algorithmic-work figure."""
 import collections
+import json
 import os
 import subprocess
 import sys
@@ -228,6 +229,23 @@ def test_cli_sym_fold_chunk(gpu, golden):
     assert r.stdout == f"{g['answer']}\n"
     r = _run([os.path.join(BIN, "sym_fold"), "6", "8"])
     assert r.stdout == f"{golden['square_totals']['6']}\n"
+
+
+def _sorted_windows():
+    with open(os.path.join(ROOT, "tests", "golden", "sym_sorted_windows.json")) as f:
+        return json.load(f)["items"]
+
+
+@pytest.mark.parametrize("item", _sorted_windows(), ids=lambda it: f"n{it['n']}-d{it['cost_depth']}-{it['sort'] or 'nosort'}")
+def test_cli_sorted_windows_match_reference(gpu, item):
+    """sym_fold's materialised mode with MF_COST_DEPTH / MF_SORT_CASES: the
+    window of the scored and sorted case list counts what the reference's
+    binary printed (tests/golden/make_sorted_windows.sh)."""
+    _say(f"cli sym_fold sorted window {item}")
+    r = _run([os.path.join(BIN, "sym_fold"), str(item["n"]), "8", str(item["max"]), str(item["start"])],
+             {"MF_COST_DEPTH": str(item["cost_depth"]), "MF_SORT_CASES": item["sort"]})
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == f"{item['answer']}\n"
 
 
 def test_graph_rounds_match_host_rounds(gpu):

@@ -129,3 +129,24 @@ def test_sym_expand_matches_oracle(mf, oracle):
     assert np.array_equal(ch[:, :13], oracle.sym_prefixes(5, 13)[:, :13])
     with pytest.raises(mf.MFError):
         mf.sym_expand(5, 24, pre)   # no children past level n^2 - 2
+
+
+def _sorted_windows():
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "sym_sorted_windows.json")) as f:
+        return json.load(f)["items"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "sym_fold")), reason="CLIs not built")
+@pytest.mark.parametrize("item", _sorted_windows(), ids=lambda it: f"n{it['n']}-d{it['cost_depth']}-{it['sort'] or 'nosort'}")
+def test_cli_cost_depth_score_line_matches_reference(item):
+    """MF_COST_DEPTH scoring (estimate_work over the materialised case list)
+    prints the reference's min/max/total scores (setup-only: no GPU needed)."""
+    r = _run([os.path.join(BIN, "sym_fold"), str(item["n"]), "8", str(item["max"]), str(item["start"])],
+             {"MF_COST_DEPTH": str(item["cost_depth"]), "MF_SORT_CASES": item["sort"], "MF_SETUP_ONLY": "1"})
+    assert r.returncode == 0, r.stderr
+    line = next(l for l in r.stderr.splitlines() if l.startswith("cost_depth="))
+    assert line.split(" score_s=")[0] == item["cost_line"]
+    sorted_line = [l for l in r.stderr.splitlines() if l.startswith("sorted_cases=")]
+    want = [] if item["sort"] in ("", "0") else [f"sorted_cases={'asc' if item['sort'] == 'asc' else 'desc'}"]
+    assert sorted_line == want
