@@ -70,6 +70,14 @@ def lib():
                                     P(C.c_uint64), P(C.c_uint64), P(C.c_uint64), P(SymStats)]
         L.orc_sym_answer.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint64, C.c_int,
                                      P(C.c_uint64), P(SymStats)]
+        L.orc9_default_center.argtypes = [C.c_int]
+        L.orc9_seeds.argtypes = [C.c_int, C.c_int, P(C.c_uint8), P(C.c_uint64), C.c_int64, P(C.c_int64)]
+        L.orc9_seeds.restype = C.c_int64
+        L.orc9_cases.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint64, P(C.c_uint8),
+                                 P(C.c_uint64)]
+        L.orc9_cases.restype = C.c_int64
+        L.orc9_solve.argtypes = [C.c_int, C.c_int, P(C.c_uint8), P(C.c_uint64), C.c_int64, C.c_int,
+                                 P(C.c_uint64), P(C.c_uint64), C.c_uint64 * 2, P(Sym9Stats)]
         _lib = L
     return _lib
 
@@ -256,3 +264,75 @@ def sym_prefixes(n, depth):
     out = np.zeros((k, 64), dtype=np.uint8)
     L.orc_sym_prefixes(n, depth, _ptr(out, C.c_uint8), k)
     return out
+
+
+# ----------------------------------------------------------------------------- 9x9 (sym_fold9.cpp)
+
+
+class Sym9Stats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("nodes", "page_walks", "walk_steps", "leaves")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+CELLS9 = 128   # bytes per 9x9-capable cycle row
+
+
+def sym9_default_center(n):
+    return int(lib().orc9_default_center(n))
+
+
+def sym9_seeds(n, c=None):
+    """(rows uint8 [k, 128], multipliers, raw center count) -- sym_fold9.cpp center dedup."""
+    L = lib()
+    c = sym9_default_center(n) if c is None else c
+    raw = C.c_int64()
+    k = L.orc9_seeds(n, c, None, None, C.c_int64(0), C.byref(raw))
+    if k < 0:
+        raise ValueError("bad n / center")
+    out = np.zeros((k, CELLS9), dtype=np.uint8)
+    mul = np.zeros(k, dtype=np.uint64)
+    L.orc9_seeds(n, c, out.ctypes.data_as(C.POINTER(C.c_uint8)), mul.ctypes.data_as(C.POINTER(C.c_uint64)),
+                 C.c_int64(k), C.byref(raw))
+    return out, mul, int(raw.value)
+
+
+def sym9_cases(n, expanded, start, max_cases, c=None):
+    L = lib()
+    c = sym9_default_center(n) if c is None else c
+    out = np.zeros((max_cases, CELLS9), dtype=np.uint8)
+    mul = np.zeros(max_cases, dtype=np.uint64)
+    k = L.orc9_cases(n, c, expanded, C.c_uint64(start), C.c_uint64(max_cases),
+                     out.ctypes.data_as(C.POINTER(C.c_uint8)), mul.ctypes.data_as(C.POINTER(C.c_uint64)))
+    if k < 0:
+        raise ValueError("bad arguments")
+    return out[:k], mul[:k]
+
+
+def sym9_solve(n, level, cycles, mults=None, threads=8):
+    """(per-case counts, per-case nodes, sum mult*count (int), stats)."""
+    L = lib()
+    cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+    k = cycles.shape[0]
+    pc = np.zeros(k, dtype=np.uint64)
+    pn = np.zeros(k, dtype=np.uint64)
+    tot = (C.c_uint64 * 2)()
+    st = Sym9Stats()
+    mp = None
+    if mults is not None:
+        mults = np.ascontiguousarray(mults, dtype=np.uint64)
+        mp = mults.ctypes.data_as(C.POINTER(C.c_uint64))
+    rc = L.orc9_solve(n, level, cycles.ctypes.data_as(C.POINTER(C.c_uint8)), mp, C.c_int64(k), threads,
+                      pc.ctypes.data_as(C.POINTER(C.c_uint64)), pn.ctypes.data_as(C.POINTER(C.c_uint64)), tot,
+                      C.byref(st))
+    if rc != 0:
+        raise ValueError(f"orc9_solve rc={rc}")
+    return pc, pn, int(tot[0]) | (int(tot[1]) << 64), st.as_dict()
+
+
+def sym9_answer(n, expanded, start, max_cases, c=None, threads=8):
+    """sym_fold9 <n> threads max start with MF_STREAM_CASES=1 MF_EXPANDED_N=expanded."""
+    cyc, mul = sym9_cases(n, expanded, start, max_cases, c)
+    _, _, tot, st = sym9_solve(n, expanded, cyc, mul, threads)
+    return tot * n * n, st

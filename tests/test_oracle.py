@@ -197,3 +197,48 @@ def test_sym_count_to_intermediate_stop_counts_prefixes(oracle):
     pre = oracle.sym_prefixes(5, 12)
     mid = sum(oracle.sym_count_to(5, c, 12, 16)[0] for c in pre)
     assert mid == oracle.sym_prefixes(5, 16).shape[0]
+
+
+# ----------------------------------------------------------------------------- 9x9 (sym_fold9.cpp)
+
+
+@pytest.fixture(scope="module")
+def sym9_golden():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "sym9.json")) as f:
+        return json.load(f)
+
+
+def test_sym9_small_squares(oracle, sym9_golden):
+    """The 128-bit path on n <= 6 reproduces the published totals."""
+    import ctypes as C
+    cyc = np.zeros(128, dtype=np.uint8)
+    cyc[:3] = [1, 2, 0]
+    cnt = (C.c_uint64 * 2)()
+    assert oracle.lib().orc9_count_case(5, cyc.ctypes.data_as(C.POINTER(C.c_uint8)), 3, cnt, None) == 0
+    assert 2 * 25 * (cnt[0] | (cnt[1] << 64)) == sym9_golden["squares"]["5"]
+    assert oracle.sym9_answer(6, 16, 0, 10**6)[0] == sym9_golden["squares"]["6"]
+
+
+def test_sym9_matches_64bit_oracle_on_7x7(oracle):
+    """Independent restatements agree: 128-bit masks vs 64-bit masks (7x7, 3x3 center)."""
+    a9, st9 = oracle.sym9_answer(7, 24, 1000, 60)
+    a, st = oracle.sym_answer(7, 24, 1000, 60)
+    assert a9 == a
+    assert st9["nodes"] == st["nodes"] and st9["walk_steps"] == st["walk_steps"]
+
+
+def test_sym9_center_seeds(oracle, sym9_golden):
+    g = sym9_golden["center_9"]
+    seeds, mult, raw = oracle.sym9_seeds(9)
+    assert raw == g["center_cases"] and len(seeds) == g["reduced_cases"]
+    hist = {str(k): int(v) for k, v in zip(*np.unique(mult, return_counts=True))}
+    assert hist == g["multiplier_hist"]
+    assert int(mult.sum()) == 2 * raw
+
+
+@pytest.mark.parametrize("idx", [0, 3])
+def test_sym9_windows(oracle, sym9_golden, idx):
+    w = sym9_golden["windows"][idx]
+    assert oracle.sym9_answer(w["n"], w["expanded"], w["start"], w["max"])[0] == w["answer"]
