@@ -18,7 +18,7 @@ schedule without the model, and the plain static-frontier kernel (one lane
 per case, one launch) on the chunk's first --plain-cases cases (it is bounded
 by its longest case: minutes on the whole chunk), are timed beside it, the
 latter also on learned and lookahead-proxy splits.  Other configurations:
---workload sym7 | lunnon6 | rect64.
+--workload sym7 | sym9 | lunnon6 | rect64.
 
     python bench.py [--steps K] [--warmup W] [--workload W] [--start S] [--cases C]
                     [--no-plain] [--no-cpu] [--cpu-cases M] [--plain-cases P]
@@ -37,7 +37,24 @@ GOLDEN_CHUNK = (120000000, 50000, 26687875796992)   # published/reference chunk 
 SYM_WORKLOADS = {
     "sym8": (8, 36, 120000000, 50000, 26687875796992),
     "sym7": (7, 24, 200000, 20000, 4968634597136),
+    # 9x9 (sym_fold9: 5x5 center seeds, u128 masks): MF_EXPANDED_N=44 window [0, 2000)
+    "sym9": (9, 44, 0, 2000, None),
 }
+# Default CPU-baseline samples (cases) per workload: about 10-30 s of host work.
+CPU_CASES = {"sym8": 4000, "sym7": 4000, "sym9": 400}
+
+
+def sym_golden(workload, start, cases):
+    """Golden answer of a sym workload window, or None."""
+    n, level, start0, cases0, golden = SYM_WORKLOADS[workload]
+    if (start, cases) != (start0, cases0):
+        return None
+    if golden is None:   # the 9x9 windows of the reference binary
+        with open(os.path.join(ROOT, "tests", "golden", "sym9.json")) as f:
+            for w in json.load(f)["windows"]:
+                if (w["n"], w["expanded"], w["start"], w["max"]) == (n, level, start, cases):
+                    return w["answer"]
+    return golden
 
 
 def log(msg):
@@ -199,14 +216,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sym8", choices=["sym8", "sym7", "lunnon6", "rect64"],
+    ap.add_argument("--workload", default="sym8", choices=["sym8", "sym7", "sym9", "lunnon6", "rect64"],
                     help="sym8 (default): 8x8 depth-36 50k-case chunk; sym7: 7x7 depth-24 20k cases; "
+                         "sym9: 9x9 depth-44 2000 cases (no learned model: its features cover n <= 8); "
                          "lunnon6: 6x6 Lunnon frontier at depth 18; rect64: 64 seeded rectangles in one queue")
     ap.add_argument("--start", type=int, default=None)
     ap.add_argument("--cases", type=int, default=None)
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain static-frontier kernel")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cases", type=int, default=4000)
+    ap.add_argument("--cpu-cases", type=int, default=None)
     ap.add_argument("--plain-cases", type=int, default=300)
     ap.add_argument("--seed", type=int, default=0, help="rect64: seed of the shape draw")
     ap.add_argument("--budget", type=int, default=0)
@@ -243,11 +261,16 @@ def main():
     if args.workload in ("lunnon6", "rect64"):
         print(json.dumps(run_lunnon(args, mf, rl)), flush=True)
         return
-    n, level, start0, cases0, golden0 = SYM_WORKLOADS[args.workload]
+    n, level, start0, cases0, _ = SYM_WORKLOADS[args.workload]
     if args.start is None:
         args.start = start0
     if args.cases is None:
         args.cases = cases0
+    if args.cpu_cases is None:
+        args.cpu_cases = CPU_CASES[args.workload]
+    if n > 8 and not args.no_learned:
+        log("9x9: the cost model's features cover n <= 8; timing the re-split schedule without it")
+        args.no_learned = True
     mf.init(0)
     cus = mf.device_cus()
     t0 = time.time()
@@ -264,7 +287,7 @@ def main():
                                                      core=args.core, graph=graph), per_item=True)
     nodes, walks, steps, leaves = res.nodes, res.walks, res.steps, res.leaves
     total = res.count * n * n
-    golden = golden0 if (args.start, args.cases) == (start0, cases0) else None
+    golden = sym_golden(args.workload, args.start, args.cases)
     if golden is not None and total != golden:
         raise SystemExit(f"chunk total {total} != golden {golden}")
     log(f"counting run: total={total} nodes={nodes} walks={walks} steps={steps} rounds={res.rounds} "
@@ -354,6 +377,7 @@ def main():
             f"items {r.items}")
 
     ms_per_step = 1e3 * sum(wall) / len(wall)
+    item_bytes = 80 if n * n <= 64 else 128
     node_rate = gpu_nodes / (ms_per_step / 1e3)
     case_rate = cyc.shape[0] / (ms_per_step / 1e3)
     kernel_s = sum(kern) / len(kern)
@@ -408,11 +432,16 @@ def main():
         m = min(args.cpu_cases, cyc.shape[0])
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         t1 = time.perf_counter()
-        _, _, csum, cst = oracle.sym_solve(n, level, cyc[:m], mul[:m], threads=threads)
+        if n <= 8:
+            _, _, csum, cst = oracle.sym_solve(n, level, cyc[:m], mul[:m], threads=threads)
+            src = "sym_fold.cpp recurse_count"
+        else:
+            _, _, csum, cst = oracle.sym9_solve(n, level, cyc[:m], mul[:m], threads=threads)
+            src = "sym_fold9.cpp recurse_count (u128 masks)"
         cs = time.perf_counter() - t1
         cpu = {"value": cst["nodes"] / cs, "unit": "nodes/s", "cores": threads, "kind": "port",
                "sample": f"first {m} cases of the chunk ({cst['nodes']} nodes), oracle C restatement of "
-                         f"sym_fold.cpp recurse_count on {threads} threads, {cs:.1f} s"}
+                         f"{src} on {threads} threads, {cs:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.3e} nodes/s on {threads} threads ({cs:.1f}s)")
 
     out = {
@@ -437,10 +466,10 @@ def main():
         "nodes": nodes, "page_walks": walks, "walk_steps": steps, "leaves": leaves,
         "rounds": rounds[-1] if rounds else None,
         "kernel_ms_per_step": kernel_s * 1e3,
-        # Every work item (80 B) is read once; every item but the inputs was
-        # written once by the round that split it.
+        # Every work item (80 B; 128 B for 9x9) is read once; every item but
+        # the inputs was written once by the round that split it.
         "items_per_step": items[-1] if items else None,
-        "algorithmic_bytes_per_launch": (80 * (2 * items[-1] - n_inputs) / rounds[-1]) if items else None,
+        "algorithmic_bytes_per_launch": (item_bytes * (2 * items[-1] - n_inputs) / rounds[-1]) if items else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s",
                      "frac": achieved / peak, "traffic": _traffic(args),
                      "ops_per_step": ops,

@@ -15,7 +15,11 @@
  *   mf_sym_seeds          sym_fold.cpp:689-710 (recurse_generate + deduplicate_cases)
  *   mf_sym_cases          sym_fold.cpp:491-527 (recurse_stream_cases, case-index window)
  *   mf_sym_solve          sym_fold.cpp:219-233,410-489,529-620 (recurse_count over cases)
- *   mf_sym_fold           sym_fold.cpp:649-774 (main: the decimal answer)
+ *   mf_sym_fold           sym_fold.cpp:649-774 (main: the decimal answer);
+ *                         for n = 9, experiments/9x9/sym_fold9.cpp:984-1198
+ *   mf_sym_cases_slice,   experiments/9x9/sym_fold9.cpp:1071-1090,1139-1172
+ *   mf_sym_fold_slice,      (MF_SEED_START / MF_SEED_COUNT seed slices and the
+ *   mf_sym_seed_case_counts  per-seed count_generated_cases of MF_SEED_COUNTS)
  *   mf_wasm_*             mfah/wasm/sym_kernel.c:20-34,251-277 (batch case ABI)
  */
 #ifndef MAPFOLD_H
@@ -134,8 +138,14 @@ int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps,
                     const mf_solve_options *opt, uint64_t *per_map_count, mf_result *out);
 
 /* ---- symmetry-reduced cycle DFS (sym_fold.cpp) --------------------------- */
-/* Seeds: deduplicated center cycles for 6 <= n <= 8 (64-byte cycles, orbit
- * multipliers), in generation order.  *raw = center cycles before dedup. */
+/* Case rows: the cycle successor of each cell, one byte per cell, in rows of
+ * 64 bytes for n <= 8 and 128 bytes for n = 9 (cells past the row's level
+ * are zero).  The seed center is c x c cells with c = 5 for n = 9, 4 for even
+ * and 3 for odd n <= 8; the environment variable MF_CENTER_N overrides it
+ * (same parity as n, 3 <= c, c*c <= 25), as in sym_fold9.cpp:968-980. */
+
+/* Seeds: deduplicated center cycles for 6 <= n <= 9 (orbit multipliers), in
+ * generation order.  *raw = center cycles before dedup. */
 int mf_sym_seeds(int32_t n, uint8_t *cycles, uint64_t *mults, int64_t max_seeds,
                  int64_t *num_seeds, int64_t *raw);
 
@@ -150,7 +160,20 @@ int mf_sym_cases(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases
 /* Total number of expanded cases at level `expanded`. */
 int mf_sym_case_total(int32_t n, int32_t expanded, uint64_t *total);
 
-/* Count `num` cases (64-byte cycles valid on cells [0, level)) on the GPU.
+/* A seed slice: seeds [seed_start, seed_start + seed_count) of the seed list
+ * (clipped to it; MF_ALL_SEEDS = every seed from seed_start).  Case index 0
+ * of a slice is the first case of seed seed_start, as with the reference's
+ * MF_SEED_START / MF_SEED_COUNT. */
+#define MF_ALL_SEEDS UINT64_MAX
+int mf_sym_cases_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count,
+                       uint64_t start, uint64_t max_cases, uint8_t *cycles, uint64_t *mults,
+                       int64_t *num);
+/* Cases at level `expanded` under each seed of the slice, in seed order.
+ * *num = the slice's seed count; counts (nullable) needs room for it. */
+int mf_sym_seed_case_counts(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count,
+                            uint64_t *counts, int64_t max_counts, int64_t *num);
+
+/* Count `num` cases (case rows valid on cells [0, level)) on the GPU.
  * mults may be NULL (all 1).  out->count = sum mult*count (not times n^2). */
 int mf_sym_solve(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults,
                  int64_t num, const mf_solve_options *opt,
@@ -160,6 +183,10 @@ int mf_sym_solve(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t
  * (expanded = MF_EXPANDED_N for n >= 7): the decimal answer as 128 bits. */
 int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
                 const mf_solve_options *opt, uint64_t answer[2], mf_result *out);
+/* mf_sym_fold over the case stream of a seed slice (sym_fold9 MF_SEED_START). */
+int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count,
+                      uint64_t start, uint64_t max_cases, const mf_solve_options *opt,
+                      uint64_t answer[2], mf_result *out);
 
 /* ---- device-resident frontiers (inputs stay in HBM across solves) ------- */
 typedef struct mf_frontier mf_frontier;

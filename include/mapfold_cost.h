@@ -72,9 +72,10 @@ int mf_cost_gradient(mf_cost_model *m, const float *x, const float *y, int64_t c
 int mf_cost_train(mf_cost_model *m, const float *x, const float *y, int64_t count,
                   const mf_cost_train_options *opt, float *epoch_loss);
 
-/* The reference's lookahead cost proxy (estimate_work, sym_fold.cpp:235-250):
- * for each case, the number of its descendants at level
- * min(n*n, level + depth) (1 when that is the case's own level). */
+/* The reference's lookahead cost proxy (estimate_work, sym_fold.cpp:235-250;
+ * sym_fold9.cpp:359-374 for n = 9): for each case (case rows as in
+ * mapfold.h: 64 bytes, 128 for n = 9), the number of its descendants at
+ * level min(n*n, level + depth) (1 when that is the case's own level). */
 int mf_sym_estimate_work(int32_t n, int32_t level, const uint8_t *cycles, int64_t count, int32_t depth,
                          uint64_t *scores);
 

@@ -28,8 +28,8 @@ from ._abi import (MF_SCHED_RESPLIT, MF_SCHED_STATIC, MFError, MFResult, MFSolve
 __all__ = [
     "MF_SCHED_STATIC", "MF_SCHED_RESPLIT", "MFError", "MFResult", "SolveOptions", "init",
     "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
-    "sym_cases", "sym_case_total", "sym_solve", "sym_fold", "SymFrontier", "LunnonFrontier",
-    "LunnonBatchFrontier", "SymPieces", "sym_expand", "lunnon_frontier_size",
+    "sym_cases", "sym_seed_case_counts", "ALL_SEEDS", "sym_case_total", "sym_solve", "sym_fold",
+    "SymFrontier", "LunnonFrontier", "LunnonBatchFrontier", "SymPieces", "sym_expand", "lunnon_frontier_size", "case_bytes",
     "declared_symbols",
 ]
 
@@ -86,6 +86,11 @@ def init(device: int = 0) -> None:
 
 def device_cus() -> int:
     return int(lib().mf_device_cus())
+
+
+def case_bytes(n: int) -> int:
+    """Bytes per sym case row: 64 up to 8x8, 128 for 9x9 (u128-mask kernels)."""
+    return 64 if n * n <= 64 else 128
 
 
 def _dims(dims):
@@ -159,24 +164,43 @@ def lunnon_batch(shapes, depths, options: SolveOptions | None = None):
 
 
 def sym_seeds(n: int):
-    """(cycles uint8 [S, 64], multipliers uint64 [S], raw center count)."""
+    """(cycles uint8 [S, case_bytes(n)], multipliers uint64 [S], raw center count)."""
     ns, raw = C.c_int64(), C.c_int64()
     check(lib().mf_sym_seeds(n, None, None, 0, C.byref(ns), C.byref(raw)), "mf_sym_seeds")
-    cyc = np.zeros((ns.value, 64), dtype=np.uint8)
+    cyc = np.zeros((ns.value, case_bytes(n)), dtype=np.uint8)
     mul = np.zeros(ns.value, dtype=np.uint64)
     check(lib().mf_sym_seeds(n, _p(cyc, C.c_uint8), _p(mul, C.c_uint64), ns.value, C.byref(ns), C.byref(raw)),
           "mf_sym_seeds")
     return cyc, mul, int(raw.value)
 
 
-def sym_cases(n: int, expanded: int, start: int, count: int):
-    """Cases [start, start+count) of the stream at level `expanded`: (cycles [k,64], mults [k])."""
-    cyc = np.zeros((count, 64), dtype=np.uint8)
+ALL_SEEDS = (1 << 64) - 1
+
+
+def sym_cases(n: int, expanded: int, start: int, count: int, seed_start: int = 0, seed_count: int | None = None):
+    """Cases [start, start+count) of the stream at level `expanded`: (cycles [k, case_bytes(n)], mults [k]).
+
+    With seed_start / seed_count the stream is that of the seed slice
+    (sym_fold9 MF_SEED_START / MF_SEED_COUNT): case 0 is seed seed_start's first."""
+    cyc = np.zeros((count, case_bytes(n)), dtype=np.uint8)
     mul = np.zeros(count, dtype=np.uint64)
     got = C.c_int64()
-    check(lib().mf_sym_cases(n, expanded, start, count, _p(cyc, C.c_uint8), _p(mul, C.c_uint64), C.byref(got)),
-          "mf_sym_cases")
+    sc = ALL_SEEDS if seed_count is None else seed_count
+    check(lib().mf_sym_cases_slice(n, expanded, seed_start, sc, start, count, _p(cyc, C.c_uint8),
+                                   _p(mul, C.c_uint64), C.byref(got)), "mf_sym_cases_slice")
     return cyc[:got.value], mul[:got.value]
+
+
+def sym_seed_case_counts(n: int, expanded: int, seed_start: int = 0, seed_count: int | None = None) -> np.ndarray:
+    """Cases at level `expanded` under each seed of the slice (uint64 [seeds])."""
+    sc = ALL_SEEDS if seed_count is None else seed_count
+    num = C.c_int64()
+    check(lib().mf_sym_seed_case_counts(n, expanded, seed_start, sc, None, 0, C.byref(num)),
+          "mf_sym_seed_case_counts")
+    out = np.zeros(num.value, dtype=np.uint64)
+    check(lib().mf_sym_seed_case_counts(n, expanded, seed_start, sc, _p(out, C.c_uint64), num.value,
+                                        C.byref(num)), "mf_sym_seed_case_counts")
+    return out
 
 
 def sym_case_total(n: int, expanded: int) -> int:
@@ -190,8 +214,12 @@ def sym_solve(n: int, level: int, cycles: np.ndarray, mults=None, options: Solve
     """Count cases on the GPU: (MFResult with count = sum mult*count, per-case counts, per-case nodes)."""
     cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
     num = cycles.shape[0]
+    if cycles.ndim != 2 or cycles.shape[1] != case_bytes(n):
+        raise ValueError(f"cycles must be [num, {case_bytes(n)}] for n={n}, got {cycles.shape}")
     if mults is not None:
         mults = np.ascontiguousarray(mults, dtype=np.uint64)
+        if mults.shape != (num,):
+            raise ValueError("mults must have one entry per case")
     pc = np.zeros(num, dtype=np.uint64) if per_item else None
     pn = np.zeros(num, dtype=np.uint64) if per_item else None
     res = MFResult()
@@ -203,13 +231,15 @@ def sym_solve(n: int, level: int, cycles: np.ndarray, mults=None, options: Solve
 
 
 def sym_fold(n: int, expanded: int = 24, start: int = 0, max_cases: int = 0,
-             options: SolveOptions | None = None):
-    """The decimal answer of `sym_fold n threads max_cases start_case` (MF_STREAM_CASES)."""
+             options: SolveOptions | None = None, seed_start: int = 0, seed_count: int | None = None):
+    """The decimal answer of `sym_fold n threads max_cases start_case` (MF_STREAM_CASES);
+    n = 9 and seed slices as `sym_fold9` (MF_SEED_START / MF_SEED_COUNT)."""
     ans = np.zeros(2, dtype=np.uint64)
     res = MFResult()
     o, keep = _opts(options)
-    check(lib().mf_sym_fold(n, expanded, start, max_cases, C.byref(o), _p(ans, C.c_uint64), C.byref(res)),
-          "mf_sym_fold")
+    sc = ALL_SEEDS if seed_count is None else seed_count
+    check(lib().mf_sym_fold_slice(n, expanded, seed_start, sc, start, max_cases, C.byref(o),
+                                  _p(ans, C.c_uint64), C.byref(res)), "mf_sym_fold_slice")
     return int(ans[0]) + (int(ans[1]) << 64), res
 
 
@@ -248,8 +278,12 @@ class SymFrontier(_Frontier):
 
     def __init__(self, n: int, level: int, cycles: np.ndarray, mults=None):
         cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+        if cycles.ndim != 2 or cycles.shape[1] != case_bytes(n):
+            raise ValueError(f"cycles must be [num, {case_bytes(n)}] for n={n}, got {cycles.shape}")
         if mults is not None:
             mults = np.ascontiguousarray(mults, dtype=np.uint64)
+            if mults.shape != (cycles.shape[0],):
+                raise ValueError("mults must have one entry per case")
         h = C.c_void_p()
         check(lib().mf_sym_frontier_create(n, level, _p(cycles, C.c_uint8), _p(mults, C.c_uint64),
                                            cycles.shape[0], C.byref(h)), "mf_sym_frontier_create")

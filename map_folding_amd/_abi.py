@@ -100,6 +100,9 @@ def lib() -> C.CDLL:
         "mf_sym_seeds": (C.c_int, [i32, P(u8), P(u64), i64, P(i64), P(i64)]),
         "mf_sym_cases": (C.c_int, [i32, i32, u64, u64, P(u8), P(u64), P(i64)]),
         "mf_sym_case_total": (C.c_int, [i32, i32, P(u64)]),
+        "mf_sym_cases_slice": (C.c_int, [i32, i32, u64, u64, u64, u64, P(u8), P(u64), P(i64)]),
+        "mf_sym_seed_case_counts": (C.c_int, [i32, i32, u64, u64, P(u64), i64, P(i64)]),
+        "mf_sym_fold_slice": (C.c_int, [i32, i32, u64, u64, u64, u64, P(MFSolveOptions), P(u64), P(MFResult)]),
         "mf_sym_solve": (C.c_int, [i32, i32, P(u8), P(u64), i64, P(MFSolveOptions), P(u64), P(u64),
                                    P(MFResult)]),
         "mf_sym_fold": (C.c_int, [i32, i32, u64, u64, P(MFSolveOptions), P(u64), P(MFResult)]),

@@ -145,8 +145,11 @@ def learned_order(pred) -> np.ndarray:
 
 def estimate_work(n: int, level: int, cycles: np.ndarray, depth: int) -> np.ndarray:
     """The reference's lookahead proxy: descendants at level min(n^2, level+depth)
-    of each case (estimate_work, sym_fold.cpp:235-250)."""
+    of each case (estimate_work, sym_fold.cpp:235-250); rows of case_bytes(n)."""
+    from . import case_bytes
     cycles = np.ascontiguousarray(cycles, dtype=np.uint8)
+    if cycles.ndim != 2 or cycles.shape[1] != case_bytes(n):
+        raise ValueError(f"cycles must be [num, {case_bytes(n)}] for n={n}")
     out = np.zeros(cycles.shape[0], dtype=np.uint64)
     check(lib().mf_sym_estimate_work(n, level, cycles.ctypes.data_as(C.POINTER(C.c_uint8)), cycles.shape[0], depth,
                                      out.ctypes.data_as(C.POINTER(C.c_uint64))), "mf_sym_estimate_work")

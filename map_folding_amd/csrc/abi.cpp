@@ -22,9 +22,9 @@
 
 namespace mf {
 hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid, uint32_t block, uint32_t lds,
-                      bool events, hipStream_t);
-uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block);
-const void *sym_kernel_ptr(int core, bool persistent, bool events);
+                      bool events, bool cells96, hipStream_t);
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, bool cells96);
+const void *sym_kernel_ptr(int core, bool persistent, bool events, bool cells96);
 hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t grid, uint32_t block,
                          uint32_t lds, bool events, hipStream_t);
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block);
@@ -164,9 +164,9 @@ void fill_result(mf_result *out, const unsigned long long *acc) {
 
 // Generic solve driver over a device-resident item array.
 //   launch(args, persistent, grid) runs one kernel launch on g.stream.
-template <typename Args, typename Launch>
+template <typename Item, typename Args, typename Launch>
 int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds_bytes, const Tuned &t,
-              const WorkItem *dev_items, uint64_t num, uint64_t origins, uint64_t *per_count, uint64_t *per_nodes,
+              const Item *dev_items, uint64_t num, uint64_t origins, uint64_t *per_count, uint64_t *per_nodes,
               uint64_t *per_map, uint32_t nmaps, mf_result *out, Launch launch) {
     const auto t0 = std::chrono::steady_clock::now();
     std::memset(out, 0, sizeof(*out));
@@ -224,11 +224,11 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
             uint64_t cap = t.capacity ? t.capacity : lanes * 65ull;
             cap = std::max<uint64_t>(cap, lanes * 2);
             if (cap > 0xffffffffull) cap = 0xffffffffull;
-            if ((rc = g.spill_a.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
-            if ((rc = g.spill_b.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
-            WorkItem *bufs[2] = {static_cast<WorkItem *>(g.spill_a.ptr), static_cast<WorkItem *>(g.spill_b.ptr)};
+            if ((rc = g.spill_a.ensure(cap * sizeof(Item))) != MF_OK) return rc;
+            if ((rc = g.spill_b.ensure(cap * sizeof(Item))) != MF_OK) return rc;
+            Item *bufs[2] = {static_cast<Item *>(g.spill_a.ptr), static_cast<Item *>(g.spill_b.ptr)};
             uint32_t *counts = claim + 1;   // counts[q]: items written to bufs[q] by the last round into it
-            const WorkItem *in = dev_items;
+            const Item *in = dev_items;
             uint64_t in_num = num;
             int next = 0;
             auto check_errors = [&]() -> int {
@@ -364,7 +364,7 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
 }  // namespace
 
 struct mf_frontier {
-    int kind = 0;                  // 1 = sym, 2 = Lunnon
+    int kind = 0;                  // 1 = sym (up to 8x8), 2 = Lunnon, 3 = sym 9x9 (u128 masks)
     SymProblem sym;
     int level = 0;
     int stop = 0;                  // sym: paths are counted at this level (n2 = full foldings)
@@ -374,6 +374,7 @@ struct mf_frontier {
     uint64_t num = 0;                   // work items
     uint64_t origins = 0;               // entries of the per-item result arrays (0 = num)
     std::vector<WorkItem> host_items;   // kept for learned reordering
+    std::vector<WorkItem128> host_items128;   // kind 3 (9x9 sym)
 };
 
 namespace {
@@ -424,14 +425,40 @@ int lunnon_args(const mf_frontier &f, LunnonArgs &a, std::vector<uint8_t> &blob,
     return MF_OK;
 }
 
-int upload_items(mf_frontier *f) {
+template <typename Item>
+int upload_vec(mf_frontier *f, const std::vector<Item> &v) {
     int rc;
-    if ((rc = f->items.ensure(std::max<size_t>(1, f->host_items.size()) * sizeof(WorkItem))) != MF_OK) return rc;
-    if (!f->host_items.empty())
-        HIP_TRY(hipMemcpyAsync(f->items.ptr, f->host_items.data(), f->host_items.size() * sizeof(WorkItem),
-                               hipMemcpyHostToDevice, g.stream));
+    if ((rc = f->items.ensure(std::max<size_t>(1, v.size()) * sizeof(Item))) != MF_OK) return rc;
+    if (!v.empty())
+        HIP_TRY(hipMemcpyAsync(f->items.ptr, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
-    f->num = f->host_items.size();
+    f->num = v.size();
+    return MF_OK;
+}
+
+int upload_items(mf_frontier *f) {
+    return f->kind == 3 ? upload_vec(f, f->host_items128) : upload_vec(f, f->host_items);
+}
+
+// The caller's order as a permutation of the host items (a repeated item would
+// be counted twice), uploaded to `dst`.
+template <typename Item>
+int reorder_items(const std::vector<Item> &src, const uint32_t *order, DevBuf &dst, const Item **dev) {
+    const uint64_t num = src.size();
+    std::vector<Item> tmp(num);
+    std::vector<uint8_t> seen(num, 0);
+    for (uint64_t k = 0; k < num; ++k) {
+        const uint32_t from = order[k];
+        if (from >= num || seen[from])
+            return fail(MF_ERR_ARG, "order is not a permutation (entry %llu)", static_cast<unsigned long long>(k));
+        seen[from] = 1;
+        tmp[k] = src[from];
+    }
+    int rc;
+    if ((rc = dst.ensure(num * sizeof(Item))) != MF_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(dst.ptr, tmp.data(), num * sizeof(Item), hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    *dev = static_cast<const Item *>(dst.ptr);
     return MF_OK;
 }
 
@@ -440,36 +467,31 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
     int rc = require_init();
     if (rc != MF_OK) return rc;
     const Tuned t = tune(opt);
-    // Learned / caller order: permute the resident items once on the host
-    // side; origins stay attached to each item so results are unchanged.
-    const WorkItem *dev_items = static_cast<const WorkItem *>(f->items.ptr);
+    // Learned / caller order: permute the resident items on the host side;
+    // origins stay attached to each item so results are unchanged.
     DevBuf reordered;
-    if (opt && opt->order && f->num > 0) {
-        // Must be a permutation: a repeated item would be counted twice.
-        std::vector<WorkItem> tmp(f->num);
-        std::vector<uint8_t> seen(f->num, 0);
-        for (uint64_t k = 0; k < f->num; ++k) {
-            const uint32_t src = opt->order[k];
-            if (src >= f->num || seen[src])
-                return fail(MF_ERR_ARG, "order is not a permutation (entry %llu)", static_cast<unsigned long long>(k));
-            seen[src] = 1;
-            tmp[k] = f->host_items[src];
-        }
-        if ((rc = reordered.ensure(f->num * sizeof(WorkItem))) != MF_OK) return rc;
-        HIP_TRY(hipMemcpyAsync(reordered.ptr, tmp.data(), f->num * sizeof(WorkItem), hipMemcpyHostToDevice, g.stream));
-        dev_items = static_cast<const WorkItem *>(reordered.ptr);
-    }
-    if (f->kind == 1) {
+    const bool ordered = opt && opt->order && f->num > 0;
+    if (f->kind == 1 || f->kind == 3) {
+        const bool c96 = f->kind == 3;
+        const int core = c96 ? 2 : t.core;
         SymArgs a;
         uint32_t levels = 0;
         sym_args(*f, a, &levels);
-        const uint32_t lds = sym_lds_bytes(t.core, levels, t.block);
+        const uint32_t lds = sym_lds_bytes(core, levels, t.block, c96);
         auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
-            return launch_sym(args, t.core, persistent, grid, t.block, lds, t.events, g.stream);
+            return launch_sym(args, core, persistent, grid, t.block, lds, t.events, c96, g.stream);
         };
-        rc = run_solve(a, sym_kernel_ptr(t.core, false, t.events), sym_kernel_ptr(t.core, true, t.events), lds, t,
-                       dev_items, f->num, f->origins ? f->origins : f->num, per_count, per_nodes, nullptr, 0, out,
-                       launch);
+        const void *ks = sym_kernel_ptr(core, false, t.events, c96), *kp = sym_kernel_ptr(core, true, t.events, c96);
+        const uint64_t origins = f->origins ? f->origins : f->num;
+        if (c96) {
+            const WorkItem128 *dev = static_cast<const WorkItem128 *>(f->items.ptr);
+            if (ordered && (rc = reorder_items(f->host_items128, opt->order, reordered, &dev)) != MF_OK) return rc;
+            rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, launch);
+        } else {
+            const WorkItem *dev = static_cast<const WorkItem *>(f->items.ptr);
+            if (ordered && (rc = reorder_items(f->host_items, opt->order, reordered, &dev)) != MF_OK) return rc;
+            rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, launch);
+        }
     } else {
         LunnonArgs a;
         std::vector<uint8_t> blob;
@@ -487,6 +509,8 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         auto launch = [&](const LunnonArgs &args, bool persistent, uint32_t grid) {
             return launch_lunnon(args, t.core, persistent, grid, t.block, lds, t.events, g.stream);
         };
+        const WorkItem *dev_items = static_cast<const WorkItem *>(f->items.ptr);
+        if (ordered && (rc = reorder_items(f->host_items, opt->order, reordered, &dev_items)) != MF_OK) return rc;
         rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events), lunnon_kernel_ptr(t.core, true, t.events), lds,
                        t, dev_items,
                        f->num, f->origins ? f->origins : f->num, per_count, per_nodes, per_map,
@@ -500,14 +524,16 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
 // A case must be one cycle through exactly the cells [0, level): then every
 // get_good walk is an orbit of a permutation and terminates.
 bool valid_cycle(const uint8_t *cycle, int level) {
-    uint64_t seen = 0;
+    using u128 = unsigned __int128;
+    if (level < 1 || level > 127) return false;
+    u128 seen = 0;
     int c = 0;
     for (int k = 0; k < level; ++k) {
         if (cycle[c] >= level) return false;
-        seen |= 1ull << c;
+        seen |= static_cast<u128>(1) << c;
         c = cycle[c];
     }
-    return c == 0 && seen == ((level >= 64) ? ~0ull : ((1ull << level) - 1ull));
+    return c == 0 && seen == ((static_cast<u128>(1) << level) - 1);
 }
 
 // A Lunnon state must be one circular list through leaves [0, depth] with
@@ -525,6 +551,16 @@ WorkItem sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origi
     it.mult = static_cast<uint16_t>(mult);
     it.level = static_cast<uint8_t>(level);
     it.map = 0;
+    return it;
+}
+
+WorkItem128 sym_item128(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
+    WorkItem128 it;
+    std::memset(&it, 0, sizeof(it));
+    std::memcpy(it.state, cycle, sizeof(it.state));
+    it.origin = origin;
+    it.mult = static_cast<uint16_t>(mult);
+    it.level = static_cast<uint8_t>(level);
     return it;
 }
 
@@ -775,38 +811,61 @@ int mf_sym_seeds(int32_t n, uint8_t *cycles, uint64_t *mults, int64_t max_seeds,
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
-    if (n < 6) return fail(MF_ERR_ARG, "seeds exist for 6 <= n <= 8");
+    if (n < 6) return fail(MF_ERR_ARG, "seeds exist for 6 <= n <= 9");
     int64_t r = 0;
-    const auto seeds = sym_seeds(pr, &r);
+    const auto &seeds = sym_seed_list(pr, &r);
     if (raw) *raw = r;
     if (num_seeds) *num_seeds = static_cast<int64_t>(seeds.size());
     const int64_t k = std::min<int64_t>(max_seeds, static_cast<int64_t>(seeds.size()));
+    const int cb = pr.case_bytes();
     for (int64_t s = 0; s < k; ++s) {
-        if (cycles) std::memcpy(cycles + s * 64, seeds[s].cycle, 64);
+        if (cycles) std::memcpy(cycles + s * cb, seeds[s].cycle, cb);
         if (mults) mults[s] = seeds[s].mult;
     }
     if ((cycles || mults) && max_seeds < static_cast<int64_t>(seeds.size())) return fail(MF_ERR_CAPACITY, "too many seeds");
     return MF_OK;
 }
 
-static int check_expanded(int n, int expanded) {
-    const int c2 = (n % 2 == 0) ? 16 : 9;
-    if (expanded < c2 || expanded > n * n - 1)
-        return fail(MF_ERR_DEPTH, "MF_EXPANDED_N must be between %d and %d", c2, n * n - 1);
+static int check_expanded(const SymProblem &pr, int expanded) {
+    const int c2 = pr.center * pr.center;
+    if (expanded < c2 || expanded > pr.n2 - 1)
+        return fail(MF_ERR_DEPTH, "MF_EXPANDED_N must be between %d and %d", c2, pr.n2 - 1);
+    return MF_OK;
+}
+
+int mf_sym_cases_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count, uint64_t start,
+                       uint64_t max_cases, uint8_t *cycles, uint64_t *mults, int64_t *num) {
+    SymProblem pr;
+    std::string err;
+    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 9");
+    int rc = check_expanded(pr, expanded);
+    if (rc != MF_OK) return rc;
+    if (max_cases > 0 && (!cycles || !mults)) return fail(MF_ERR_ARG, "output buffers are NULL");
+    const int64_t got = sym_cases(pr, expanded, start, max_cases, cycles, mults, seed_start, seed_count);
+    if (num) *num = got;
     return MF_OK;
 }
 
 int mf_sym_cases(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases, uint8_t *cycles, uint64_t *mults,
                  int64_t *num) {
+    return mf_sym_cases_slice(n, expanded, 0, MF_ALL_SEEDS, start, max_cases, cycles, mults, num);
+}
+
+int mf_sym_seed_case_counts(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count, uint64_t *counts,
+                            int64_t max_counts, int64_t *num) {
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
-    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 8");
-    int rc = check_expanded(n, expanded);
+    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 9");
+    int rc = check_expanded(pr, expanded);
     if (rc != MF_OK) return rc;
-    if (max_cases > 0 && (!cycles || !mults)) return fail(MF_ERR_ARG, "output buffers are NULL");
-    const int64_t got = sym_cases(pr, expanded, start, max_cases, cycles, mults);
-    if (num) *num = got;
+    const auto got = sym_slice_case_counts(pr, expanded, seed_start, seed_count);
+    if (num) *num = static_cast<int64_t>(got.size());
+    if (!counts) return MF_OK;
+    if (max_counts < static_cast<int64_t>(got.size())) return fail(MF_ERR_CAPACITY, "counts has room for %lld of %zu seeds",
+                                                                 static_cast<long long>(max_counts), got.size());
+    std::memcpy(counts, got.data(), got.size() * sizeof(uint64_t));
     return MF_OK;
 }
 
@@ -814,8 +873,8 @@ int mf_sym_case_total(int32_t n, int32_t expanded, uint64_t *total) {
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
-    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 8");
-    int rc = check_expanded(n, expanded);
+    if (n < 6) return fail(MF_ERR_ARG, "the case stream exists for 6 <= n <= 9");
+    int rc = check_expanded(pr, expanded);
     if (rc != MF_OK) return rc;
     const auto &counts = sym_seed_case_counts(pr, expanded);
     uint64_t t = 0;
@@ -844,20 +903,28 @@ static int sym_frontier_create_stop(int32_t n, int32_t level, int32_t stop, cons
     if (n < 2) return fail(MF_ERR_ARG, "n must be at least 2 for the cycle DFS");
     if (stop < 2 || stop > pr.n2) return fail(MF_ERR_DEPTH, "stop level must be between 2 and %d", pr.n2);
     if (level < 1 || level > stop - 1) return fail(MF_ERR_DEPTH, "level must be between 1 and %d", stop - 1);
+    const int cb = pr.case_bytes();
     for (int64_t s = 0; s < num; ++s) {
         const uint64_t m = mults ? mults[s] : 1;
         if (m == 0 || m > 0xffff) return fail(MF_ERR_ARG, "multiplier out of range at case %lld", static_cast<long long>(s));
-        if (!valid_cycle(cycles + s * 64, level))
+        if (!valid_cycle(cycles + s * cb, level))
             return fail(MF_ERR_ARG, "case %lld is not one cycle through cells 0..%d", static_cast<long long>(s), level - 1);
     }
     auto *f = new mf_frontier();
-    f->kind = 1;
+    f->kind = pr.n2 <= 64 ? 1 : 3;
     f->sym = pr;
     f->level = level;
     f->stop = stop;
-    f->host_items.reserve(static_cast<size_t>(num));
-    for (int64_t s = 0; s < num; ++s)
-        f->host_items.push_back(sym_item(cycles + s * 64, mults ? mults[s] : 1, level, static_cast<uint32_t>(s)));
+    if (f->kind == 1) {
+        f->host_items.reserve(static_cast<size_t>(num));
+        for (int64_t s = 0; s < num; ++s)
+            f->host_items.push_back(sym_item(cycles + s * cb, mults ? mults[s] : 1, level, static_cast<uint32_t>(s)));
+    } else {
+        f->host_items128.reserve(static_cast<size_t>(num));
+        for (int64_t s = 0; s < num; ++s)
+            f->host_items128.push_back(
+                sym_item128(cycles + s * cb, mults ? mults[s] : 1, level, static_cast<uint32_t>(s)));
+    }
     if ((rc = upload_items(f)) != MF_OK) {
         delete f;
         return rc;
@@ -885,6 +952,7 @@ int mf_sym_expand(int32_t n, int32_t level, const uint8_t *cycles, int64_t count
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (n > 8) return fail(MF_ERR_ARG, "split frontiers take n <= 8");
     if (level < 1 || level > pr.n2 - 2) return fail(MF_ERR_DEPTH, "level must be between 1 and %d", pr.n2 - 2);
     int64_t w = 0;
     for (int64_t s = 0; s < count; ++s) {
@@ -918,7 +986,7 @@ int mf_sym_pieces_create(int32_t n, const uint8_t *cycles, const uint8_t *levels
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
-    if (n < 2) return fail(MF_ERR_ARG, "n must be at least 2 for the cycle DFS");
+    if (n < 2 || n > 8) return fail(MF_ERR_ARG, "split frontiers take 2 <= n <= 8");
     int lo = pr.n2;
     for (int64_t s = 0; s < num; ++s) {
         const int level = levels[s];
@@ -951,9 +1019,14 @@ int mf_sym_pieces_create(int32_t n, const uint8_t *cycles, const uint8_t *levels
 
 int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases, const mf_solve_options *opt,
                 uint64_t answer[2], mf_result *out) {
+    return mf_sym_fold_slice(n, expanded, 0, MF_ALL_SEEDS, start, max_cases, opt, answer, out);
+}
+
+int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t seed_count, uint64_t start,
+                      uint64_t max_cases, const mf_solve_options *opt, uint64_t answer[2], mf_result *out) {
     if (!answer || !out) return fail(MF_ERR_ARG, "answer/out is NULL");
     std::memset(out, 0, sizeof(*out));
-    if (n < 1 || n > 8) return fail(MF_ERR_ARG, "n must be between 1 and 8");
+    if (n < 1 || n > 9) return fail(MF_ERR_ARG, "n must be between 1 and 9");
     unsigned __int128 ans = 0;
     if (n == 1) {
         ans = 1;
@@ -967,14 +1040,14 @@ int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
         if (rc != MF_OK) return rc;
         ans = static_cast<unsigned __int128>(2) * static_cast<unsigned>(n * n) * out->count_lo;
     } else {
-        const int level = (n >= 7) ? expanded : 16;
-        if (n >= 7) {
-            int rc = check_expanded(n, expanded);
-            if (rc != MF_OK) return rc;
-        }
         SymProblem pr;
         std::string err;
-        sym_problem(n, pr, err);
+        if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+        const int level = (n >= 7) ? expanded : pr.center * pr.center;
+        if (n >= 7) {
+            int rc = check_expanded(pr, expanded);
+            if (rc != MF_OK) return rc;
+        }
         // Stream the window in batches that bound host and device memory.
         const uint64_t batch = 1ull << 22;
         uint64_t pos = start, left = max_cases == 0 ? ~0ull : max_cases;
@@ -984,9 +1057,9 @@ int mf_sym_fold(int32_t n, int32_t expanded, uint64_t start, uint64_t max_cases,
         mf_result part;
         while (left > 0) {
             const uint64_t want = std::min(batch, left);
-            cyc.resize(want * 64);
+            cyc.resize(want * pr.case_bytes());
             mul.resize(want);
-            const int64_t got = sym_cases(pr, level, pos, want, cyc.data(), mul.data());
+            const int64_t got = sym_cases(pr, level, pos, want, cyc.data(), mul.data(), seed_start, seed_count);
             if (got <= 0) break;
             int rc = mf_sym_solve(n, level, cyc.data(), mul.data(), got, opt, nullptr, nullptr, &part);
             if (rc != MF_OK) return rc;
@@ -1050,9 +1123,9 @@ static uint32_t wasm_write(unsigned __int128 count, uint64_t nodes, uint32_t sta
 }
 
 uint32_t mf_wasm_solve_cases(uint32_t n, uint32_t start_i, uint32_t stop_i, uint32_t case_count) {
-    // 64-bit masks: squares up to 8x8 (the reference's wasm build also takes 9x9).
-    if (n < 2 || n > 8 || case_count > 256) {
-        fail(MF_ERR_ARG, "n must be 2..8 and case_count <= 256");
+    // Squares up to 9x9 (MAX_SIDE): u64-mask kernels to 8x8, u128 for 9x9.
+    if (n < 2 || n > 9 || case_count > 256) {
+        fail(MF_ERR_ARG, "n must be 2..9 and case_count <= 256");
         return wasm_write(0, 0, 1);
     }
     const uint32_t n2 = n * n;
@@ -1060,12 +1133,13 @@ uint32_t mf_wasm_solve_cases(uint32_t n, uint32_t start_i, uint32_t stop_i, uint
         fail(MF_ERR_DEPTH, "need start_i <= stop_i <= n*n");
         return wasm_write(0, 0, 2);
     }
-    std::vector<uint8_t> cycles(static_cast<size_t>(case_count) * 64, 0);
+    const size_t cb = n2 <= 64 ? 64 : 128;   // case row bytes
+    std::vector<uint8_t> cycles(static_cast<size_t>(case_count) * cb, 0);
     std::vector<uint64_t> mults(case_count);
     unsigned __int128 total = 0;
     for (uint32_t k = 0; k < case_count; ++k) {
         const uint8_t *rec = g_wasm_cases + k * 112;
-        std::memcpy(cycles.data() + k * 64, rec, n2);
+        std::memcpy(cycles.data() + k * cb, rec, n2);
         uint64_t m = 0;
         for (int b = 7; b >= 0; --b) m = (m << 8) | rec[100 + b];
         mults[k] = m;

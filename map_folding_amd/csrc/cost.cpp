@@ -296,18 +296,39 @@ void lunnon_row(const LunnonProblem &pr, int depth, const uint8_t *state, float 
     for (int k = 0; k < kProfile; ++k) f[72 + k] = lg(prof[k]) / 24.0f;
 }
 
-// estimate_work (sym_fold.cpp:235-250): descendants of the node at level i
-// that reach level stop.
+// estimate_work (sym_fold.cpp:235-250, sym_fold9.cpp:359-374): descendants
+// of the node at level i that reach level stop.  M = uint64_t up to 8x8,
+// unsigned __int128 for 9x9.
+template <typename M>
+M good_of(const SymProblem &pr, const uint8_t *cyc, int i);
+template <>
+uint64_t good_of<uint64_t>(const SymProblem &pr, const uint8_t *cyc, int i) { return sym_good(pr, cyc, i); }
+template <>
+unsigned __int128 good_of<unsigned __int128>(const SymProblem &pr, const uint8_t *cyc, int i) {
+    return sym_good128(pr, cyc, i);
+}
+inline int lowest(uint64_t v) { return __builtin_ctzll(v); }
+inline int lowest(unsigned __int128 v) {
+    const uint64_t lo = static_cast<uint64_t>(v);
+    return lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll(static_cast<uint64_t>(v >> 64));
+}
+inline uint64_t bits(uint64_t v) { return static_cast<uint64_t>(__builtin_popcountll(v)); }
+inline uint64_t bits(unsigned __int128 v) {
+    return static_cast<uint64_t>(__builtin_popcountll(static_cast<uint64_t>(v)) +
+                                 __builtin_popcountll(static_cast<uint64_t>(v >> 64)));
+}
+
+template <typename M>
 uint64_t descendants(const SymProblem &pr, uint8_t *cyc, int i, int stop) {
     if (i >= stop) return 1;
-    const uint64_t good = sym_good(pr, cyc, i);
-    if (i + 1 == stop) return static_cast<uint64_t>(__builtin_popcountll(good));
+    const M good = good_of<M>(pr, cyc, i);
+    if (i + 1 == stop) return bits(good);
     uint64_t out = 0;
-    for (uint64_t gm = good; gm; gm &= gm - 1) {
-        const int j = __builtin_ctzll(gm);
+    for (M gm = good; gm; gm &= gm - 1) {
+        const int j = lowest(gm);
         cyc[i] = cyc[j];
         cyc[j] = static_cast<uint8_t>(i);
-        out += descendants(pr, cyc, i + 1, stop);
+        out += descendants<M>(pr, cyc, i + 1, stop);
         cyc[j] = cyc[i];
     }
     return out;
@@ -408,6 +429,7 @@ int mf_cost_features_sym(int32_t n, int32_t level, const uint8_t *cycles, int64_
     SymProblem pr;
     std::string err;
     if (!sym_problem(n, pr, err)) return lib_error(MF_ERR_ARG, err.c_str());
+    if (n > 8) return lib_error(MF_ERR_ARG, "cost features take n <= 8");
     if (level < 1 || level >= pr.n2) return lib_error(MF_ERR_DEPTH, "level must be in [1, n*n)");
     parallel_rows(count, [&](int64_t k) { sym_row(pr, level, cycles + 64 * k, out + kFeatures * k); });
     return MF_OK;
@@ -433,10 +455,12 @@ int mf_sym_estimate_work(int32_t n, int32_t level, const uint8_t *cycles, int64_
     if (!sym_problem(n, pr, err)) return lib_error(MF_ERR_ARG, err.c_str());
     if (level < 1 || level > pr.n2 || depth < 0) return lib_error(MF_ERR_DEPTH, "need 1 <= level <= n*n, depth >= 0");
     const int stop = std::min(pr.n2, level + depth);
+    const int cb = pr.case_bytes();
     parallel_rows(count, [&](int64_t k) {
-        uint8_t cyc[kCells];
-        std::memcpy(cyc, cycles + 64 * k, kCells);
-        scores[k] = descendants(pr, cyc, level, stop);
+        uint8_t cyc[kMaxCells] = {};
+        std::memcpy(cyc, cycles + static_cast<int64_t>(cb) * k, cb);
+        scores[k] = pr.n2 <= 64 ? descendants<uint64_t>(pr, cyc, level, stop)
+                                : descendants<unsigned __int128>(pr, cyc, level, stop);
     });
     return MF_OK;
 }

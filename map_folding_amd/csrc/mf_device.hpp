@@ -26,6 +26,19 @@ struct alignas(16) WorkItem {
 };
 static_assert(sizeof(WorkItem) == 80, "WorkItem layout");
 
+// 9x9 symmetry cases: 81-cell cycles and u128 sibling masks.
+struct alignas(16) WorkItem128 {
+    uint8_t state[96];
+    uint64_t mask_lo;
+    uint64_t mask_hi;
+    uint32_t origin;
+    uint16_t mult;
+    uint8_t level;
+    uint8_t map;       // floor of a recompute piece (0 = none)
+    uint8_t pad[8];
+};
+static_assert(sizeof(WorkItem128) == 128, "WorkItem128 layout");
+
 // Global accumulators (device memory, zeroed before a solve).
 enum AccSlot {
     ACC_COUNT_LO = 0,  // sum of mult * count, 128 bits
@@ -53,8 +66,8 @@ struct LunnonMap {
 constexpr int kMaxMaps = 64;
 
 struct SymArgs {
-    uint32_t edge_word[64];     // 4 pages of cell c packed into one word (partner, n2 = none)
-    const WorkItem *items;
+    uint32_t edge_word[96];     // 4 pages of cell c packed into one word (partner, n2 = none)
+    const void *items;          // WorkItem (up to 64 cells) or WorkItem128 (9x9)
     uint32_t num_items;
     const uint32_t *num_items_dev;  // persistent: when set, the item count is read here (graph rounds)
     uint32_t last;              // n2 - 1: the level whose good mask is counted
@@ -63,7 +76,7 @@ struct SymArgs {
     uint32_t budget;            // persistent: core steps between claim/spill checks
     uint32_t tail;              // persistent: steps a lane may run after the input is drained
     uint32_t *claim;            // persistent: next input index
-    WorkItem *spill;            // persistent: output queue of split work
+    void *spill;                // persistent: output queue of split work (item type as `items`)
     uint32_t *spill_count;
     uint32_t spill_cap;
     unsigned long long *acc;    // AccSlot

@@ -25,10 +25,13 @@ __device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ?
 __device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) { return lane_base + (c << 6); }
 __device__ __forceinline__ uint32_t row_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
 
+// Copy a work item's first kBytes state bytes (a multiple of 16) into the
+// lane's row, and back.
+template <uint32_t kBytes = 64>
 __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (uint32_t q = 0; q < kBytes / 16; ++q) {
         const uint4 v = s4[q];
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -36,15 +39,28 @@ __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_
     }
 }
 
+template <uint32_t kBytes = 64>
 __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
     uint4 *d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (uint32_t q = 0; q < kBytes / 16; ++q) {
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 16; ++k) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
         d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
+}
+
+// Lowest set bit of a u64 or u128 mask (nonzero).
+__device__ __forceinline__ uint32_t mask_ctz(uint64_t v) { return static_cast<uint32_t>(__builtin_ctzll(v)); }
+__device__ __forceinline__ uint32_t mask_ctz(unsigned __int128 v) {
+    const uint64_t lo = static_cast<uint64_t>(v);
+    return lo ? static_cast<uint32_t>(__builtin_ctzll(lo))
+              : 64u + static_cast<uint32_t>(__builtin_ctzll(static_cast<uint64_t>(v >> 64)));
+}
+__device__ __forceinline__ uint32_t mask_popcount(uint64_t v) { return static_cast<uint32_t>(__popcll(v)); }
+__device__ __forceinline__ uint32_t mask_popcount(unsigned __int128 v) {
+    return static_cast<uint32_t>(__popcll(static_cast<uint64_t>(v)) + __popcll(static_cast<uint64_t>(v >> 64)));
 }
 
 struct Lane {
@@ -63,6 +79,7 @@ struct WideStack {
     static constexpr uint32_t kSlotBytes = 512;   // u64 x 64 lanes
     static constexpr uint32_t kLaneBytes = 8;
     using Entry = uint64_t;
+    using MaskT = uint64_t;
     static __device__ __forceinline__ uint64_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
         return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
     }
@@ -82,43 +99,51 @@ struct WideStack {
     }
 };
 
-// 16 bits per level: bits 0-5 current child j, bits 6-11 next sibling,
-// bits 12-13 state: 0 = j is the last child, 1 = the next sibling is the
-// last, 2 = more follow the next sibling, 3 = more follow j but are not
-// recorded (the node is recomputed to find them).
-struct CompactStack {
+// 16 bits per level: bits [0, B) current child j, [B, 2B) next sibling, then
+// 2 state bits: 0 = j is the last child, 1 = the next sibling is the last,
+// 2 = more follow the next sibling, 3 = more follow j but are not recorded
+// (the node is recomputed to find them).  B = 6 for up to 64 cells (u64
+// masks), 7 for up to 128 (u128 masks).
+template <typename Mask, uint32_t B>
+struct CompactStackT {
+    static_assert(2 * B + 2 <= 16, "entry must fit 16 bits");
     static constexpr uint32_t kSlotBytes = 128;   // u16 x 64 lanes
     static constexpr uint32_t kLaneBytes = 2;
+    static constexpr uint32_t kField = (1u << B) - 1u;
     using Entry = uint32_t;
+    using MaskT = Mask;
     static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
         return *reinterpret_cast<const uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
     }
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
         *reinterpret_cast<uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = static_cast<uint16_t>(e);
     }
-    static __device__ __forceinline__ uint32_t encode(uint64_t g) {
-        const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(g));
-        const uint64_t r = g & (g - 1);
+    static __device__ __forceinline__ uint32_t encode(Mask g) {
+        const uint32_t j = mask_ctz(g);
+        const Mask r = g & (g - 1);
         if (r == 0) return j;
-        const uint32_t j2 = static_cast<uint32_t>(__builtin_ctzll(r));
+        const uint32_t j2 = mask_ctz(r);
         const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
-        return j | (j2 << 6) | (state << 12);
+        return j | (j2 << B) | (state << (2 * B));
     }
-    static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & 63u; }
+    static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & kField; }
     // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.
     static __device__ __forceinline__ uint32_t more(uint32_t e) {
-        const uint32_t s = e >> 12;
+        const uint32_t s = e >> (2 * B);
         return s == 0 ? 0u : (s == 3 ? 2u : 1u);
     }
     static __device__ __forceinline__ uint32_t advance(uint32_t e) {
-        const uint32_t j2 = (e >> 6) & 63u;
-        return j2 | ((e >> 12) == 2u ? (3u << 12) : 0u);
+        const uint32_t j2 = (e >> B) & kField;
+        return j2 | ((e >> (2 * B)) == 2u ? (3u << (2 * B)) : 0u);
     }
-    static __device__ __forceinline__ uint64_t rest(uint32_t e, bool &unknown) {
-        const uint32_t s = e >> 12;
+    static __device__ __forceinline__ Mask rest(uint32_t e, bool &unknown) {
+        const uint32_t s = e >> (2 * B);
         unknown = s >= 2u;
-        return s == 1u ? (1ull << ((e >> 6) & 63u)) : 0ull;
+        return s == 1u ? (static_cast<Mask>(1) << ((e >> B) & kField)) : static_cast<Mask>(0);
     }
 };
+
+using CompactStack = CompactStackT<uint64_t, 6>;
+using CompactStack128 = CompactStackT<unsigned __int128, 7>;
 
 }  // namespace mf

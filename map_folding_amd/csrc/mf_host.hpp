@@ -15,7 +15,8 @@ namespace mf {
 
 constexpr int kMaxLeaves = 64;   // n = prod(dims) <= 64 (one u64 mask per level)
 constexpr int kMaxDims = 4;      // GPU Lunnon kernel: up to 4 dimensions
-constexpr int kCells = 64;       // sym: up to 8x8 cells, one u64 mask
+constexpr int kCells = 64;       // sym: case row bytes up to 8x8 (one u64 mask)
+constexpr int kMaxCells = 128;   // sym: case row bytes for 9x9 (81 cells, u128 masks)
 
 // ------------------------------------------------------------------ Lunnon
 
@@ -41,30 +42,47 @@ int64_t lunnon_frontier(const LunnonProblem &pr, int depth, uint8_t *out, int64_
 struct SymProblem {
     int n = 0;
     int n2 = 0;
-    uint8_t edge[4][kCells];      // edge[p][c] = neighbour of cell c on page p, n2 if none
-    uint32_t edge_word[kCells];   // the 4 pages of cell c packed, page p in bits 8p..8p+7
-    int map[kCells];              // spiral number of (x, y) at map[x*n + y]
+    int center = 0;                  // c: side of the seed center (c*c cells)
+    uint8_t edge[4][kMaxCells];      // edge[p][c] = neighbour of cell c on page p, n2 if none
+    uint32_t edge_word[kMaxCells];   // the 4 pages of cell c packed, page p in bits 8p..8p+7
+    int map[kMaxCells];              // spiral number of (x, y) at map[x*n + y]
+    // Bytes per case row: 64 up to 8x8, 128 for 9x9.
+    int case_bytes() const { return n2 <= 64 ? kCells : kMaxCells; }
 };
 
+// n in 1..9.  The seed center is c = 5 for n = 9, else 4 (n even) or 3 (n
+// odd), overridden by MF_CENTER_N as in sym_fold9.cpp (same parity as n,
+// c*c <= 25).
 bool sym_problem(int n, SymProblem &pr, std::string &err);
 
-// Host get_good (used by the generators only).
+// Host get_good (generators, features).  sym_good: n <= 8; sym_good128: any n.
 uint64_t sym_good(const SymProblem &pr, const uint8_t *cycle, int i);
+unsigned __int128 sym_good128(const SymProblem &pr, const uint8_t *cycle, int i);
 
 struct SymSeed {
-    uint8_t cycle[kCells];
+    uint8_t cycle[kMaxCells];
     uint64_t mult;
 };
 
 // Deduplicated center cycles in generation order; raw = number before dedup.
 std::vector<SymSeed> sym_seeds(const SymProblem &pr, int64_t *raw);
+// The same, computed once per (n, center) in the process.
+const std::vector<SymSeed> &sym_seed_list(const SymProblem &pr, int64_t *raw);
 
-// Per-seed case counts at level `expanded` (cached per (n, expanded) in the
-// process; computed on all host threads).
+// Per-seed case counts at level `expanded` (cached per (n, center, expanded)
+// in the process; computed on all host threads).
 const std::vector<uint64_t> &sym_seed_case_counts(const SymProblem &pr, int expanded);
 
-// Cases [start, start+max) in stream order.  Returns the number written.
+// Per-seed case counts of the seed slice [seed_start, seed_start+seed_count)
+// (clipped to the seed list; the full list comes from the cache above).
+std::vector<uint64_t> sym_slice_case_counts(const SymProblem &pr, int expanded, uint64_t seed_start,
+                                            uint64_t seed_count);
+
+// Cases [start, start+max) in stream order of the seed slice [seed_start,
+// seed_start+seed_count) (sym_fold9.cpp:1071-1090: case index 0 is the first
+// case of seed seed_start), pr.case_bytes() bytes per case.  Returns the
+// number written.
 int64_t sym_cases(const SymProblem &pr, int expanded, uint64_t start, uint64_t max_cases,
-                  uint8_t *cycles, uint64_t *mults);
+                  uint8_t *cycles, uint64_t *mults, uint64_t seed_start = 0, uint64_t seed_count = ~0ull);
 
 }  // namespace mf

@@ -46,10 +46,39 @@
 namespace mf {
 namespace {
 
-constexpr uint32_t kPartnerOff = 0;      // 64 words: byte p = partner on page p (n2 = none)
-constexpr uint32_t kPageOff = 256;       // the same, page-major: byte p*64 + c
-constexpr uint32_t kTableBytes = 512;
-constexpr uint32_t kCycleBytes = kRowBytes;
+// Cell geometry of a launch: up to 64 cells (u64 masks, 80-byte items) or up
+// to 96 (9x9: 81 cells, u128 masks, 128-byte items).  LDS: a partner word per
+// cell (byte p = partner on page p, n2 = none), the same page-major (byte
+// kPageOff + kCells*p + c), then per wave a kCells-entry byte row per lane and
+// the stack.
+template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_>
+struct Geometry {
+    static constexpr uint32_t kCells = kCells_;
+    using Mask = Mask_;
+    using Item = Item_;
+    using Stack = Stack_;
+    static constexpr uint32_t kPartnerOff = 0;
+    static constexpr uint32_t kPageOff = 4 * kCells;
+    static constexpr uint32_t kTableBytes = 8 * kCells;
+    static constexpr uint32_t kCycleBytes = kCells * 64;       // one row of kCells bytes per lane
+    static constexpr uint32_t kStateBytes = kCells;
+    static __device__ __forceinline__ Mask below(uint32_t i) {
+        return i >= 8 * sizeof(Mask) ? ~static_cast<Mask>(0) : ((static_cast<Mask>(1) << i) - 1);
+    }
+    static __device__ __forceinline__ Mask item_mask(const WorkItem *it) { return it->mask; }
+    static __device__ __forceinline__ Mask item_mask(const WorkItem128 *it) {
+        return (static_cast<Mask>(it->mask_hi) << 64) | it->mask_lo;
+    }
+    static __device__ __forceinline__ void set_item_mask(WorkItem *it, Mask m) { it->mask = static_cast<uint64_t>(m); }
+    static __device__ __forceinline__ void set_item_mask(WorkItem128 *it, Mask m) {
+        it->mask_lo = static_cast<uint64_t>(m);
+        it->mask_hi = static_cast<uint64_t>(static_cast<unsigned __int128>(m) >> 64);
+    }
+};
+
+using G64 = Geometry<64, uint64_t, WorkItem, CompactStack>;
+using G64Wide = Geometry<64, uint64_t, WorkItem, WideStack>;
+using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128>;
 
 __device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_at(lane_base, cell); }
 
@@ -84,26 +113,29 @@ struct Totals {
     unsigned long long nodes = 0, walks = 0, steps = 0, leaves = 0, items = 0;
 };
 
-// get_good (sym_fold.cpp:179-204) for cell i against this lane's cycle.
-__device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
-                                              uint32_t &walks, uint32_t &steps) {
-    uint64_t good = below_mask(i);
-    const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + kPartnerOff)[i];
+// get_good (sym_fold.cpp:179-204; u128 masks as sym_fold9.cpp:286-311) for
+// cell i against this lane's cycle.
+template <typename G>
+__device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
+                                                      uint32_t &walks, uint32_t &steps) {
+    using Mask = typename G::Mask;
+    Mask good = G::below(i);
+    const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + G::kPartnerOff)[i];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const uint32_t anchor = (ew >> (8 * p)) & 0xffu;
         if (anchor < i && good != 0) {
             ++walks;
-            uint64_t reach = 1ull << anchor;
+            Mask reach = static_cast<Mask>(1) << anchor;
             uint32_t j = lds[cyc_addr(cl, anchor)];
             while (j != anchor) {
                 ++steps;
                 // Absolute LDS address (the kernel's only LDS is the dynamic
                 // block at offset 0): the page offset folds into the ds offset.
-                const uint32_t partner =
-                    *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(kPageOff + 64u * p + j);
+                const uint32_t partner = *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(
+                    G::kPageOff + G::kCells * p + j);
                 j = (partner < i) ? partner : j;
-                reach |= 1ull << j;
+                reach |= static_cast<Mask>(1) << j;
                 j = lds[cyc_addr(cl, j)];
             }
             good &= reach;
@@ -123,11 +155,12 @@ __device__ __forceinline__ uint64_t good_mask(const uint8_t *__restrict__ lds, u
 #endif
 constexpr uint32_t kMaxPops = MF_SYM_MAX_POPS;
 
-template <typename Stack, bool EV>
+template <typename G, bool EV>
 __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t last, Work &w) {
+    using Stack = typename G::Stack;
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
-        uint64_t g = good_mask(lds, ln.cl, w.i, nw, ns);
+        typename G::Mask g = good_mask<G>(lds, ln.cl, w.i, nw, ns);
         if (w.floor == 0) {
             ++w.inodes;
             if (EV) {
@@ -135,11 +168,11 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
                 w.isteps += ns;
             }
             if (w.i == last) {
-                w.leaves += static_cast<uint32_t>(__builtin_popcountll(g));
+                w.leaves += mask_popcount(g);
                 g = 0;
             }
         } else {
-            g &= ~below_mask(w.floor);
+            g &= ~G::below(w.floor);
             w.floor = 0;
         }
         if (g != 0) {
@@ -183,9 +216,10 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
 // Item kinds: mask != 0: enter the listed children of node `level`;
 // mask == 0 and map == 0: expand node `level` (counted); mask == 0 and
 // map == f > 0: recompute node `level` and enter its children above f-1.
-__device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it,
-                                                 Work &w) {
-    row_load(lds, ln.cl, it->state);
+template <typename G>
+__device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln,
+                                                 const typename G::Item *__restrict__ it, Work &w) {
+    row_load<G::kStateBytes>(lds, ln.cl, it->state);
     w.origin = it->origin;
     w.mult = it->mult;
     w.root = it->level;
@@ -198,10 +232,12 @@ __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln, c
     w.popping = false;
 }
 
-template <typename Stack>
-__device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const WorkItem *__restrict__ it, Work &w) {
-    load_item_common(lds, ln, it, w);
-    const uint64_t mask = it->mask;
+template <typename G>
+__device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const typename G::Item *__restrict__ it,
+                                          Work &w) {
+    using Stack = typename G::Stack;
+    load_item_common<G>(lds, ln, it, w);
+    const typename G::Mask mask = G::item_mask(it);
     const uint32_t level = it->level;
     if (mask == 0) {
         w.i = level;
@@ -243,28 +279,29 @@ __device__ __forceinline__ void flush_wave(const SymArgs &a, uint32_t lane, cons
     }
 }
 
-template <typename Stack>
+template <typename G>
 __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t wave_bytes = kCycleBytes + levels * Stack::kSlotBytes;
-    const uint32_t wbase = kTableBytes + wave * wave_bytes;
+    const uint32_t wave_bytes = G::kCycleBytes + levels * G::Stack::kSlotBytes;
+    const uint32_t wbase = G::kTableBytes + wave * wave_bytes;
     Lane ln;
     ln.cl = row_lane_base(wbase, lane);
-    ln.sl = wbase + kCycleBytes + lane * Stack::kLaneBytes;
+    ln.sl = wbase + G::kCycleBytes + lane * G::Stack::kLaneBytes;
     ln.base = base_level;
     return ln;
 }
 
+template <typename G>
 __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
     const uint32_t tid = threadIdx.x;
     // The walk addresses the page table by absolute LDS offset.
     const auto *local = (const __attribute__((address_space(3))) uint8_t *)lds;
     if (tid == 0 && static_cast<uint32_t>(reinterpret_cast<size_t>(local)) != 0u) atomicOr(a.acc + ACC_ERROR, 4ull);
-    if (tid < 64) {
+    if (tid < G::kCells) {
         const uint32_t w = a.edge_word[tid];
-        reinterpret_cast<uint32_t *>(lds + kPartnerOff)[tid] = w;
+        reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = w;
 #pragma unroll
-        for (uint32_t p = 0; p < 4; ++p) lds[kPageOff + 64u * p + tid] = static_cast<uint8_t>(w >> (8u * p));
+        for (uint32_t p = 0; p < 4; ++p) lds[G::kPageOff + G::kCells * p + tid] = static_cast<uint8_t>(w >> (8u * p));
     }
     __syncthreads();
 }
@@ -273,12 +310,14 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
 // expanded, then for each stack level, deepest first, the siblings not yet
 // entered (a recorded sibling as a mask, unrecorded ones as a recompute piece
 // with a floor).  Called by the whole wave.
-template <typename Stack>
+template <typename G>
 __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const Lane &ln, uint32_t lane, Work &w,
                                           Totals &tot) {
+    using Stack = typename G::Stack;
+    using Item = typename G::Item;
     bool pend = w.have;
     uint32_t t = w.i;
-    uint64_t mask = 0;
+    typename G::Mask mask = 0;
     uint32_t floor = w.floor;   // the pending expansion may itself be a recompute
     // A lane in the middle of backtracking has no pending node: levels >= i
     // are finished, so it only continues the unwind, emitting siblings.
@@ -288,9 +327,9 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
         if (emit) {
             if (slot < a.spill_cap) {
-                WorkItem *dst = a.spill + slot;
-                row_store(lds, ln.cl, dst->state);
-                dst->mask = mask;
+                Item *dst = static_cast<Item *>(a.spill) + slot;
+                row_store<G::kStateBytes>(lds, ln.cl, dst->state);
+                G::set_item_mask(dst, mask);
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.mult);
                 dst->level = static_cast<uint8_t>(t);
@@ -330,19 +369,20 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
 }  // namespace
 
 // ---------------------------------------------------------------- static
-template <typename Stack, bool EV>
+template <typename G, bool EV>
 __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    stage_tables(a, lds);
-    const Lane ln = make_lane<Stack>(a.levels, a.base_level);
+    stage_tables<G>(a, lds);
+    const Lane ln = make_lane<G>(a.levels, a.base_level);
+    const auto *items = static_cast<const typename G::Item *>(a.items);
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     Totals tot;
     Work w;
     w.have = false;
     w.popping = false;
-    if (idx < a.num_items) load_item<Stack>(lds, ln, a.items + idx, w);
+    if (idx < a.num_items) load_item<G>(lds, ln, items + idx, w);
     while (w.have) {
-        if (dfs_step<Stack, EV>(lds, ln, a.last, w)) {
+        if (dfs_step<G, EV>(lds, ln, a.last, w)) {
             retire(a, w, tot);
             w.have = false;
         }
@@ -351,7 +391,7 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
 }
 
 // ------------------------------------------------------------- persistent
-template <typename Stack, bool EV>
+template <typename G, bool EV>
 __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     // Graph rounds read their item count from the previous round's spill
@@ -366,9 +406,10 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.acc + ACC_ROUNDS, 1ull);
-    stage_tables(a, lds);
+    stage_tables<G>(a, lds);
     const uint32_t lane = threadIdx.x & 63u;
-    const Lane ln = make_lane<Stack>(a.levels, a.base_level);
+    const Lane ln = make_lane<G>(a.levels, a.base_level);
+    const auto *items = static_cast<const typename G::Item *>(a.items);
     const unsigned long long below = (1ull << lane) - 1ull;
 
     Totals tot;
@@ -391,7 +432,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                 if (got + want >= num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < num_items) load_item<Stack>(lds, ln, a.items + idx, w);
+                    if (idx < num_items) load_item<G>(lds, ln, items + idx, w);
                 }
             } else {
                 // Every lane is busy: look at the shared cursor anyway, so that
@@ -402,7 +443,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
         if (!__any(w.have)) break;
 
         for (uint32_t s = 0; s < a.budget && w.have; ++s) {
-            if (dfs_step<Stack, EV>(lds, ln, a.last, w)) {
+            if (dfs_step<G, EV>(lds, ln, a.last, w)) {
                 retire(a, w, tot);
                 w.have = false;
             }
@@ -414,12 +455,12 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                 // Progress guarantee: a split piece restarts the pending
                 // expansion, so every lane first completes one counted node.
                 while (__any(w.have && w.inodes == 0)) {
-                    if (w.have && w.inodes == 0 && dfs_step<Stack, EV>(lds, ln, a.last, w)) {
+                    if (w.have && w.inodes == 0 && dfs_step<G, EV>(lds, ln, a.last, w)) {
                         retire(a, w, tot);
                         w.have = false;
                     }
                 }
-                spill_all<Stack>(a, lds, ln, lane, w, tot);
+                spill_all<G>(a, lds, ln, lane, w, tot);
                 break;
             }
         }
@@ -429,18 +470,20 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
 
 // Host-visible entry points (C++ linkage inside the library).
 namespace {
-template <typename Stack>
+template <typename G>
 const void *pick(bool persistent, bool events) {
-    if (persistent) return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<Stack, true>)
-                                  : reinterpret_cast<const void *>(&sym_persistent_kernel<Stack, false>);
-    return events ? reinterpret_cast<const void *>(&sym_static_kernel<Stack, true>)
-                  : reinterpret_cast<const void *>(&sym_static_kernel<Stack, false>);
+    if (persistent) return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<G, true>)
+                                  : reinterpret_cast<const void *>(&sym_persistent_kernel<G, false>);
+    return events ? reinterpret_cast<const void *>(&sym_static_kernel<G, true>)
+                  : reinterpret_cast<const void *>(&sym_static_kernel<G, false>);
 }
 }  // namespace
 
-// core: 1 = wide u64 stack, otherwise the compact 16-bit stack.
-const void *sym_kernel_ptr(int core, bool persistent, bool events) {
-    return core == 1 ? pick<WideStack>(persistent, events) : pick<CompactStack>(persistent, events);
+// core: 1 = wide u64 stack, otherwise the compact 16-bit stack; cells96: the
+// 9x9 geometry (compact stack only).
+const void *sym_kernel_ptr(int core, bool persistent, bool events, bool cells96) {
+    if (cells96) return pick<G96>(persistent, events);
+    return core == 1 ? pick<G64Wide>(persistent, events) : pick<G64>(persistent, events);
 }
 
 // Between graph rounds: rewind the claim cursor and empty the next spill
@@ -458,14 +501,16 @@ hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t 
 }
 
 hipError_t launch_sym(const SymArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,
-                      uint32_t lds_bytes, bool events, hipStream_t s) {
+                      uint32_t lds_bytes, bool events, bool cells96, hipStream_t s) {
     void *args[] = {const_cast<SymArgs *>(&a)};
-    return hipLaunchKernel(sym_kernel_ptr(core, persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
+    return hipLaunchKernel(sym_kernel_ptr(core, persistent, events, cells96), dim3(grid), dim3(block), args,
+                           lds_bytes, s);
 }
 
-uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block) {
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, bool cells96) {
+    if (cells96) return G96::kTableBytes + (block / 64u) * (G96::kCycleBytes + levels * G96::Stack::kSlotBytes);
     const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
-    return kTableBytes + (block / 64u) * (kCycleBytes + levels * slot);
+    return G64::kTableBytes + (block / 64u) * (G64::kCycleBytes + levels * slot);
 }
 
 }  // namespace mf

@@ -2,7 +2,7 @@
 # Golden vectors for the 9x9-capable symmetry DFS (experiments/9x9/sym_fold9.cpp):
 # the reference's binary built from its own source (`make -C oracle ref` ->
 # oracle/_ref/sym_fold9), run with MF_STREAM_CASES=1 on a few windows.
-# Writes tests/golden/sym9.json.  About 2 minutes on 8 cores.
+# Writes tests/golden/sym9.json.  About 4 minutes on 8 cores.
 set -euo pipefail
 cd "$(dirname "$0")/../.."
 BIN=oracle/_ref/sym_fold9
@@ -23,6 +23,7 @@ win 9 64 300 123456
 win 9 50 1000 0
 win 9 48 200 0
 win 9 44 50 0
+win 9 44 2000 0
 n5=$("$BIN" 5 8 2>/dev/null)
 n6=$("$BIN" 6 8 2>/dev/null)
 setup=$(MF_SETUP_ONLY=1 "$BIN" 9 8 2>&1 >/dev/null | grep -E '^center_size=' | sed 's/center_size=\([0-9]*\) center_cases=\([0-9]*\)/\1 \2/')

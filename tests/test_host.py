@@ -1,7 +1,9 @@
 """CPU tests of the product's host half: frontier generators, case stream,
 library exports and the drop-in CLIs' argument handling (no GPU compute)."""
 import ctypes
+import json
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -61,6 +63,38 @@ def test_sym_case_stream_matches_oracle(mf, oracle, n, e, start, count):
     c2, m2 = oracle.sym_cases(n, e, start, count)
     assert c1.shape == c2.shape
     assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+
+
+def test_sym9_seeds_match_oracle(mf, oracle):
+    c1, m1, r1 = mf.sym_seeds(9)
+    c2, m2, r2 = oracle.sym9_seeds(9)
+    assert r1 == r2 == 3721732
+    assert c1.shape == (465883, 128)
+    assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+
+
+@pytest.mark.parametrize("e,start,count,center", [(64, 123456, 300, None), (30, 1000, 500, None),
+                                                  (20, 100, 200, 3), (26, 5000, 300, 5)])
+def test_sym9_case_stream_matches_oracle(mf, oracle, monkeypatch, e, start, count, center):
+    if center is not None:
+        monkeypatch.setenv("MF_CENTER_N", str(center))
+    c1, m1 = mf.sym_cases(9, e, start, count)
+    c2, m2 = oracle.sym9_cases(9, e, start, count, c=center)
+    assert c1.shape == c2.shape == (count, 128)
+    assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+
+
+def test_sym9_seed_slice_is_an_offset_window(mf, oracle):
+    """Case k of the seed slice starting at seed s is case (cases under seeds
+    before s) + k of the whole stream."""
+    off = int(mf.sym_seed_case_counts(9, 30, 0, 100).sum())
+    c1, m1 = mf.sym_cases(9, 30, 5, 100, seed_start=100, seed_count=50)
+    c2, m2 = oracle.sym9_cases(9, 30, off + 5, 100)
+    assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
+    # a slice ends at its last seed
+    per = mf.sym_seed_case_counts(9, 30, 100, 3)
+    c3, _ = mf.sym_cases(9, 30, 0, 10 ** 6, seed_start=100, seed_count=3)
+    assert c3.shape[0] == int(per.sum())
 
 
 def test_sym_case_totals_golden(mf, golden):
@@ -150,3 +184,36 @@ def test_cli_cost_depth_score_line_matches_reference(item):
     sorted_line = [l for l in r.stderr.splitlines() if l.startswith("sorted_cases=")]
     want = [] if item["sort"] in ("", "0") else [f"sorted_cases={'asc' if item['sort'] == 'asc' else 'desc'}"]
     assert sorted_line == want
+
+
+def _clean(text):
+    out = []
+    for line in text.splitlines():
+        if "total_s=" in line or line.startswith(("progress=", "stream selected=")):
+            continue
+        out.append(re.sub(r" \w*_s=\S+", "", line))
+    return out
+
+
+def _sym9_host_runs():
+    with open(os.path.join(ROOT, "tests", "golden", "sym9_cli.json")) as f:
+        runs = json.load(f)["runs"]
+    return [r for r in runs if r["rc"] != 0 or "MF_SETUP_ONLY" in r["env"]]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "sym_fold9")), reason="CLIs not built")
+@pytest.mark.parametrize("run", _sym9_host_runs(), ids=lambda r: " ".join(r["args"]) + " " +
+                         " ".join(f"{k}={v}" for k, v in sorted(r["env"].items())))
+def test_cli_sym_fold9_host_side_matches_reference(run):
+    """sym_fold9 runs that end before any counting (MF_SETUP_ONLY, argument
+    errors): stdout, exit status and the clock-free stderr lines of the
+    reference binary (tests/golden/make_sym9_cli.py)."""
+    env = {k: v for k, v in os.environ.items() if not k.startswith("MF_")}
+    env.update(run["env"])
+    env["HIP_VISIBLE_DEVICES"] = ""
+    r = subprocess.run([os.path.join(BIN, "sym_fold9")] + run["args"], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == run["rc"], r.stderr
+    assert r.stdout == run["stdout"]
+    if run["rc"] == 0:
+        assert _clean(r.stderr) == run["stderr"]

@@ -40,6 +40,8 @@ def test_wasm_constants_and_status_codes(mf):
     assert L.mf_wasm_case_stride() == 112 and L.mf_wasm_max_cases() == 256
     assert L.mf_wasm_solve_cases(1, 3, 4, 1) == 1        # bad n (sym_kernel.c:252-255)
     assert _read_out(mf) == (0, 0, 1)
+    assert L.mf_wasm_solve_cases(10, 3, 4, 1) == 1       # n above MAX_SIDE = 9
+    assert L.mf_wasm_solve_cases(9, 3, 82, 1) == 2       # stop > 81
     assert L.mf_wasm_solve_cases(6, 3, 4, 257) == 1      # too many cases
     assert L.mf_wasm_solve_cases(6, 10, 4, 1) == 2       # start > stop (:256-259)
     assert L.mf_wasm_solve_cases(6, 3, 37, 1) == 2       # stop > n^2
