@@ -41,7 +41,9 @@ SYM_WORKLOADS = {
     "sym9": (9, 44, 0, 2000, None),
 }
 # Default CPU-baseline samples (cases) per workload: about 10-30 s of host work.
-CPU_CASES = {"sym8": 4000, "sym7": 4000, "sym9": 400}
+CPU_CASES = {"sym8": 4000, "sym7": 4000, "sym9": 2000, "lunnon6": 4000, "rect64": 4000}
+# Default sample of the plain static kernel (bounded by its longest case).
+PLAIN_CASES = {"sym8": 300, "sym7": 300, "sym9": 100}
 
 
 def sym_golden(workload, start, cases):
@@ -225,7 +227,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain static-frontier kernel")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cases", type=int, default=None)
-    ap.add_argument("--plain-cases", type=int, default=300)
+    ap.add_argument("--plain-cases", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0, help="rect64: seed of the shape draw")
     ap.add_argument("--budget", type=int, default=0)
     ap.add_argument("--tail", type=int, default=0)
@@ -258,6 +260,8 @@ def main():
     import map_folding_amd as mf
     from map_folding_amd import roofline as rl
 
+    if args.cpu_cases is None:
+        args.cpu_cases = CPU_CASES[args.workload]
     if args.workload in ("lunnon6", "rect64"):
         print(json.dumps(run_lunnon(args, mf, rl)), flush=True)
         return
@@ -266,8 +270,8 @@ def main():
         args.start = start0
     if args.cases is None:
         args.cases = cases0
-    if args.cpu_cases is None:
-        args.cpu_cases = CPU_CASES[args.workload]
+    if args.plain_cases is None:
+        args.plain_cases = PLAIN_CASES[args.workload]
     if n > 8 and not args.no_learned:
         log("9x9: the cost model's features cover n <= 8; timing the re-split schedule without it")
         args.no_learned = True

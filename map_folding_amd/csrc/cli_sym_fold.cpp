@@ -20,10 +20,12 @@
 // by multiplier ascending) over the generation order, the same algorithm and
 // comparisons as the reference's host code.
 //
-// sym_fold9 only (sym_fold9.cpp:968-980,1040-1172): MF_CENTER_N (center side,
-// default 5 for n = 9), MF_CENTER_HIST, MF_SEED_START / MF_SEED_COUNT (a seed
-// slice; case 0 is the first case of seed MF_SEED_START), MF_SEED_COUNTS /
-// MF_SEED_COUNTS_SUMMARY with MF_SETUP_ONLY, and MF_PROFILE_WORK (per-case
+// sym_fold9 only (sym_fold9.cpp:160-229,968-980,1040-1172): MF_CENTER_N
+// (center side, default 5 for n = 9), MF_ORDER / MF_LOCK_CENTER_N (ring-wise
+// cell numberings of odd squares), MF_CENTER_HIST, MF_SEED_START /
+// MF_SEED_COUNT (a seed slice; case 0 is the first case of seed
+// MF_SEED_START), MF_SEED_COUNTS / MF_SEED_COUNTS_SUMMARY with
+// MF_SETUP_ONLY, and MF_PROFILE_WORK (per-case
 // nodes / leaves / weighted leaves of the window, counted on the GPU; a node
 // is one get_good call as in recurse_count_work, sym_fold9.cpp:342-357).
 // MF_CENTER_STREAM_DEDUP is accepted: it changes how the reference holds the
@@ -214,8 +216,12 @@ int main(int argc, char **argv) {
         usage(argv[0]);
         return 1;
     }
-    // The 8x8 tool knows no MF_CENTER_N: keep the library on its default center.
-    if (!kNine) unsetenv("MF_CENTER_N");
+    // The 8x8 tool knows no MF_CENTER_N / MF_ORDER: keep the library on its
+    // default center and the spiral numbering.
+    if (!kNine) {
+        unsetenv("MF_CENTER_N");
+        unsetenv("MF_ORDER");
+    }
     const int n = mfcli::parse_int(argv[1], "n");
     if (n < 1 || n > MF_CLI_MAX_N) {
         std::cerr << "n must be between 1 and " << MF_CLI_MAX_N << "\n";

@@ -57,6 +57,67 @@ void spiral(int w, int h, int *map) {
     }
 }
 
+// Cell numbering (build_map, sym_fold9.cpp:160-229): the spiral above or,
+// for odd squares under MF_ORDER, a ring-wise renumbering.  Rings of
+// Chebyshev radius r around the center come inner first, each in spiral
+// order, except that rings beyond the locked radius reorder: reverse;
+// opposite (first and second half interleaved, a leftover last); axes_first
+// (stable by Manhattan distance from the center); corners_first (the four
+// corners first, stable).  The spiral itself is this construction with every
+// ring kept in order, since spiral numbers grow ring by ring.
+void cell_map(int w, int h, int order, int lock_radius, int *map) {
+    spiral(w, h, map);
+    if (order == 0 || w != h || w % 2 == 0) return;
+    const int n = w, c = n / 2;
+    std::vector<std::pair<int, int>> coords(n * n), ordered;
+    for (int x = 0; x < n; ++x)
+        for (int y = 0; y < n; ++y) coords[map[x * n + y]] = {x, y};
+    for (int r = 0; r <= c; ++r) {
+        std::vector<std::pair<int, int>> ring;
+        for (const auto &xy : coords)
+            if (std::max(std::abs(xy.first - c), std::abs(xy.second - c)) == r) ring.push_back(xy);
+        if (r > lock_radius) {
+            if (order == 1) {
+                std::reverse(ring.begin(), ring.end());
+            } else if (order == 2) {
+                std::vector<std::pair<int, int>> mixed;
+                const size_t half = ring.size() / 2;
+                for (size_t k = 0; k < half; ++k) {
+                    mixed.push_back(ring[k]);
+                    if (k + half < ring.size()) mixed.push_back(ring[k + half]);
+                }
+                if (ring.size() % 2 != 0) mixed.push_back(ring.back());
+                ring = std::move(mixed);
+            } else if (order == 3) {
+                std::stable_sort(ring.begin(), ring.end(), [c](const auto &a, const auto &b) {
+                    return std::abs(a.first - c) + std::abs(a.second - c) < std::abs(b.first - c) + std::abs(b.second - c);
+                });
+            } else if (order == 4) {
+                auto corner = [c, r](const std::pair<int, int> &a) {
+                    return std::abs(a.first - c) == r && std::abs(a.second - c) == r ? 0 : 1;
+                };
+                std::stable_sort(ring.begin(), ring.end(),
+                                 [&](const auto &a, const auto &b) { return corner(a) < corner(b); });
+            }
+        }
+        ordered.insert(ordered.end(), ring.begin(), ring.end());
+    }
+    for (int k = 0; k < n * n; ++k) map[ordered[k].first * n + ordered[k].second] = k;
+}
+
+// MF_ORDER (unset, empty, "spiral" or an unknown name: the spiral) and
+// MF_LOCK_CENTER_N (default 3: rings of radius <= (3-1)/2 keep spiral order).
+void order_from_env(int &order, int &lock_radius) {
+    order = 0;
+    lock_radius = 1;
+    const char *text = std::getenv("MF_ORDER");
+    if (text == nullptr) return;
+    const std::string o = text;
+    order = o == "reverse" ? 1 : o == "opposite" ? 2 : o == "axes_first" ? 3 : o == "corners_first" ? 4 : 0;
+    const char *lock = std::getenv("MF_LOCK_CENTER_N");
+    lock_radius = std::max(0, ((lock ? std::atoi(lock) : 3) - 1) / 2);
+}
+
 using u128 = unsigned __int128;
 
 template <typename M>
@@ -110,8 +171,10 @@ bool sym_problem(int n, SymProblem &pr, std::string &err) {
         }
         if (n >= 6) pr.center = c;
     }
+    order_from_env(pr.order, pr.lock_radius);
+    if (pr.order == 0 || n % 2 == 0) pr.order = pr.lock_radius = 0;   // numbering is the spiral
     std::memset(pr.edge, pr.n2, sizeof(pr.edge));
-    spiral(n, n, pr.map);
+    cell_map(n, n, pr.order, pr.lock_radius, pr.map);
     for (int x = 0; x < n; ++x) {
         for (int y = 0; y < n; ++y) {
             const int c = pr.map[x * n + y];
@@ -155,11 +218,13 @@ Key center_key(const uint8_t *cycle, int c2) {
     return k;
 }
 
-// The 8 symmetries of the c x c center, as maps on spiral numbers, in the
-// reference's order: identity, transpose, y-flip, rotations and flips.
-std::vector<std::array<uint8_t, 25>> center_symmetries(int c) {
+// The 8 symmetries of the c x c center, as maps on cell numbers (the c x c
+// grid numbered as the whole board is, generate_permutations,
+// sym_fold9.cpp:506-537), in the reference's order: identity, transpose,
+// y-flip, rotations and flips.
+std::vector<std::array<uint8_t, 25>> center_symmetries(const SymProblem &pr, int c) {
     int map[25];
-    spiral(c, c, map);
+    cell_map(c, c, pr.order, pr.lock_radius, map);
     int px[25], py[25];
     for (int x = 0; x < c; ++x)
         for (int y = 0; y < c; ++y) {
@@ -275,7 +340,8 @@ struct KeyHash {
 };
 
 std::mutex g_seed_mu;
-std::map<std::pair<int, int>, std::pair<std::vector<SymSeed>, int64_t>> g_seed_sets;   // (n, center)
+// (n, center, order, lock radius) -> (seeds, raw centers)
+std::map<std::tuple<int, int, int, int>, std::pair<std::vector<SymSeed>, int64_t>> g_seed_sets;
 
 }  // namespace
 
@@ -298,7 +364,7 @@ std::vector<SymSeed> sym_seeds(const SymProblem &pr, int64_t *raw) {
     const std::vector<Key> centers = pr.n2 <= 64 ? all_centers<uint64_t>(pr, c2) : all_centers<u128>(pr, c2);
     if (raw) *raw = static_cast<int64_t>(centers.size());
 
-    const auto syms = center_symmetries(c);
+    const auto syms = center_symmetries(pr, c);
     std::vector<Key> canon(centers.size());
     std::vector<uint8_t> orbit_size(centers.size());
     constexpr size_t kBlock = 4096;
@@ -346,7 +412,7 @@ std::vector<SymSeed> sym_seeds(const SymProblem &pr, int64_t *raw) {
 
 const std::vector<SymSeed> &sym_seed_list(const SymProblem &pr, int64_t *raw) {
     std::lock_guard<std::mutex> lock(g_seed_mu);
-    const auto key = std::make_pair(pr.n, pr.center);
+    const auto key = std::make_tuple(pr.n, pr.center, pr.order, pr.lock_radius);
     auto it = g_seed_sets.find(key);
     if (it == g_seed_sets.end()) {
         int64_t r = 0;
@@ -380,7 +446,7 @@ uint64_t count_below(const SymProblem &pr, uint8_t *cycle, int i, int stop, uint
 }
 
 std::mutex g_index_mu;
-std::map<std::tuple<int, int, int>, std::vector<uint64_t>> g_index;   // (n, center, expanded)
+std::map<std::tuple<int, int, int, int, int>, std::vector<uint64_t>> g_index;   // (n, center, order, lock, expanded)
 
 const std::vector<SymSeed> &cached_seeds(const SymProblem &pr) { return sym_seed_list(pr, nullptr); }
 
@@ -441,7 +507,7 @@ void clip_slice(size_t nseeds, uint64_t seed_start, uint64_t seed_count, size_t 
 const std::vector<uint64_t> &sym_seed_case_counts(const SymProblem &pr, int expanded) {
     std::lock_guard<std::mutex> lock(g_index_mu);
     const auto &seeds = cached_seeds(pr);
-    auto &counts = g_index[std::make_tuple(pr.n, pr.center, expanded)];
+    auto &counts = g_index[std::make_tuple(pr.n, pr.center, pr.order, pr.lock_radius, expanded)];
     extend_counts(pr, seeds, 0, seeds.size(), expanded, ~0ull, counts);
     return counts;
 }
@@ -527,7 +593,7 @@ int64_t sym_cases(const SymProblem &pr, int expanded, uint64_t start, uint64_t m
         seeds = &cached_seeds(pr);
         clip_slice(seeds->size(), seed_start, seed_count, first, end);
         if (first == 0 && end == seeds->size()) {
-            auto &cached = g_index[std::make_tuple(pr.n, pr.center, expanded)];
+            auto &cached = g_index[std::make_tuple(pr.n, pr.center, pr.order, pr.lock_radius, expanded)];
             extend_counts(pr, *seeds, 0, end, expanded, start, cached);
             counts = cached;
         }

@@ -43,6 +43,8 @@ struct SymProblem {
     int n = 0;
     int n2 = 0;
     int center = 0;                  // c: side of the seed center (c*c cells)
+    int order = 0;                   // cell numbering: 0 spiral, else MF_ORDER's ring order (odd n)
+    int lock_radius = 0;             // rings up to this radius keep spiral order (MF_LOCK_CENTER_N)
     uint8_t edge[4][kMaxCells];      // edge[p][c] = neighbour of cell c on page p, n2 if none
     uint32_t edge_word[kMaxCells];   // the 4 pages of cell c packed, page p in bits 8p..8p+7
     int map[kMaxCells];              // spiral number of (x, y) at map[x*n + y]
@@ -52,7 +54,8 @@ struct SymProblem {
 
 // n in 1..9.  The seed center is c = 5 for n = 9, else 4 (n even) or 3 (n
 // odd), overridden by MF_CENTER_N as in sym_fold9.cpp (same parity as n,
-// c*c <= 25).
+// c*c <= 25).  Cells are numbered by the center-out spiral, or for odd n by
+// MF_ORDER / MF_LOCK_CENTER_N's ring order (sym_fold9.cpp:160-229).
 bool sym_problem(int n, SymProblem &pr, std::string &err);
 
 // Host get_good (generators, features).  sym_good: n <= 8; sym_good128: any n.

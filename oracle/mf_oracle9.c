@@ -29,8 +29,9 @@ typedef struct {
     uint8_t edge[4][C9_CELLS];
 } orc9_problem;
 
-/* build_map (sym_fold9.cpp, as sym_fold.cpp:122-145): center-out spiral. */
-static void o9_build_map(int w, int h, int *map) {
+/* build_spiral_map (sym_fold9.cpp:123-146, as sym_fold.cpp:122-145): numbers
+ * count down from w*h-1 along a walk from corner (0,0), so the center is 0. */
+static void o9_spiral_map(int w, int h, int *map) {
     for (int k = 0; k < w * h; k++) map[k] = w * h;
     int x = 0, y = 0, dx = 1, dy = 0;
     for (int i = w * h - 1; i >= 0; --i) {
@@ -47,6 +48,67 @@ static void o9_build_map(int w, int h, int *map) {
             y = y1;
         }
     }
+}
+
+static int o9_iabs(int v) { return v < 0 ? -v : v; }
+
+/* build_map (sym_fold9.cpp:160-229): the spiral, or for odd squares with
+ * MF_ORDER = reverse | opposite | axes_first | corners_first the rings of
+ * Chebyshev radius r = 0..w/2, each in spiral order, those beyond radius
+ * (MF_LOCK_CENTER_N - 1) / 2 (default 3) reordered; cells are numbered in
+ * that sequence.  Any other MF_ORDER value keeps every ring in spiral order. */
+static void o9_build_map(int w, int h, int *map) {
+    o9_spiral_map(w, h, map);
+    const char *ot = getenv("MF_ORDER");
+    if (ot == NULL || ot[0] == 0 || strcmp(ot, "spiral") == 0 || w != h || w % 2 == 0) return;
+    const char *lt = getenv("MF_LOCK_CENTER_N");
+    int lock_radius = ((lt ? atoi(lt) : 3) - 1) / 2;
+    if (lock_radius < 0) lock_radius = 0;
+    const int center = w / 2, n2 = w * h;
+    int bx[C9_CELLS], by[C9_CELLS];
+    for (int x = 0; x < w; x++)
+        for (int y = 0; y < h; y++) {
+            bx[map[x * h + y]] = x;
+            by[map[x * h + y]] = y;
+        }
+    int ox[C9_CELLS], oy[C9_CELLS], no = 0;
+    for (int r = 0; r <= center; r++) {
+        int rx[C9_CELLS], ry[C9_CELLS], k = 0;
+        for (int q = 0; q < n2; q++) {
+            const int ax = o9_iabs(bx[q] - center), ay = o9_iabs(by[q] - center);
+            if ((ax > ay ? ax : ay) == r) {
+                rx[k] = bx[q];
+                ry[k] = by[q];
+                k++;
+            }
+        }
+        int tx[C9_CELLS], ty[C9_CELLS], m = 0;
+        if (r > lock_radius && strcmp(ot, "reverse") == 0) {
+            for (int q = k - 1; q >= 0; q--) { tx[m] = rx[q]; ty[m] = ry[q]; m++; }
+        } else if (r > lock_radius && strcmp(ot, "opposite") == 0) {
+            const int half = k / 2;
+            for (int q = 0; q < half; q++) {
+                tx[m] = rx[q]; ty[m] = ry[q]; m++;
+                if (q + half < k) { tx[m] = rx[q + half]; ty[m] = ry[q + half]; m++; }
+            }
+            if (k % 2) { tx[m] = rx[k - 1]; ty[m] = ry[k - 1]; m++; }
+        } else if (r > lock_radius && strcmp(ot, "axes_first") == 0) {
+            /* stable by Manhattan distance: take distances in increasing order */
+            for (int d = 0; d <= 2 * center; d++)
+                for (int q = 0; q < k; q++)
+                    if (o9_iabs(rx[q] - center) + o9_iabs(ry[q] - center) == d) { tx[m] = rx[q]; ty[m] = ry[q]; m++; }
+        } else if (r > lock_radius && strcmp(ot, "corners_first") == 0) {
+            for (int pass = 0; pass < 2; pass++)
+                for (int q = 0; q < k; q++) {
+                    const int corner = o9_iabs(rx[q] - center) == r && o9_iabs(ry[q] - center) == r;
+                    if (corner == (pass == 0)) { tx[m] = rx[q]; ty[m] = ry[q]; m++; }
+                }
+        } else {
+            for (int q = 0; q < k; q++) { tx[m] = rx[q]; ty[m] = ry[q]; m++; }
+        }
+        for (int q = 0; q < m; q++) { ox[no] = tx[q]; oy[no] = ty[q]; no++; }
+    }
+    for (int q = 0; q < n2; q++) map[ox[q] * h + oy[q]] = q;
 }
 
 /* make_edges (as sym_fold.cpp:147-173): page x%2 / 2 + y%2, n*n = none. */

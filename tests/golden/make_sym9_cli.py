@@ -3,7 +3,8 @@
 The reference binary is built from its own source by `make -C oracle ref`
 (oracle/_ref/sym_fold9) and run on a handful of invocations that exercise
 what only the 9x9 tool has: seed slices (MF_SEED_START / MF_SEED_COUNT),
-MF_CENTER_N, per-case work profiles (MF_PROFILE_WORK), the cost-sorted
+MF_CENTER_N, MF_ORDER / MF_LOCK_CENTER_N cell numberings, per-case work
+profiles (MF_PROFILE_WORK), the cost-sorted
 materialised mode with a 5x5 center, and raw_count for n <= 5.  stdout is kept
 whole; stderr is kept without the lines and fields that carry wall-clock
 times.  Writes tests/golden/sym9_cli.json; about 20 minutes on 8 cores (most
@@ -28,6 +29,16 @@ RUNS = [
                                "MF_EXPANDED_N": "30"}),
     (["9", "8", "2000", "10"], {"MF_CENTER_N": "3", "MF_STREAM_CASES": "1", "MF_EXPANDED_N": "40"}),
     (["5", "8"], {}),
+    # MF_ORDER / MF_LOCK_CENTER_N: ring-wise cell numberings of odd squares
+    (["9", "8", "300", "1000"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "62", "MF_ORDER": "reverse"}),
+    (["9", "8", "300", "1000"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "62", "MF_ORDER": "axes_first",
+                                 "MF_LOCK_CENTER_N": "5"}),
+    (["7", "8", "1000", "0"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "24", "MF_ORDER": "corners_first"}),
+    (["7", "8", "200", "1000"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "28", "MF_ORDER": "corners_first"}),
+    (["7", "8", "500", "2000"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "30", "MF_ORDER": "opposite"}),
+    (["7", "8", "50", "0"], {"MF_ORDER": "corners_first", "MF_COST_DEPTH": "2", "MF_SORT_CASES": "desc",
+                             "MF_EXPANDED_N": "22"}),
+    (["5", "8"], {"MF_ORDER": "corners_first"}),
     (["9", "8"], {"MF_SETUP_ONLY": "1", "MF_CENTER_HIST": "1"}),
     # setup-only runs (no counting: the host side of the tool)
     (["9", "8"], {"MF_SETUP_ONLY": "1", "MF_STREAM_CASES": "1", "MF_EXPANDED_N": "30", "MF_SEED_START": "100",
@@ -43,6 +54,8 @@ RUNS = [
                   "MF_SEED_START": "10", "MF_SEED_COUNT": "3", "MF_SEED_COUNTS": "1"}),
     (["9", "4"], {"MF_SETUP_ONLY": "1", "MF_CENTER_N": "3", "MF_STREAM_CASES": "1", "MF_EXPANDED_N": "20"}),
     (["6", "4"], {"MF_SETUP_ONLY": "1", "MF_SEED_START": "3"}),
+    (["9", "8"], {"MF_SETUP_ONLY": "1", "MF_STREAM_CASES": "1", "MF_EXPANDED_N": "28", "MF_ORDER": "opposite",
+                  "MF_SEED_START": "400000", "MF_SEED_COUNT": "30", "MF_SEED_COUNTS": "1"}),
     (["9", "4"], {"MF_CENTER_N": "4"}),
     (["9", "4"], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": "44", "MF_PROFILE_WORK": "1"}),
 ]

@@ -63,14 +63,20 @@ def test_sym9_per_case_and_events(gpu, oracle, sched, e, start, m):
         st["nodes"], st["page_walks"], st["walk_steps"], st["leaves"])
 
 
-def test_sym9_seed_slice_additivity(gpu):
+def test_sym9_seed_slice_windows(gpu):
+    """Seed slices (MF_SEED_START / MF_SEED_COUNT): windows are additive, and
+    a window that crosses from one seed into the next is the two seeds'
+    single-seed windows put together.  Seed 1001 holds 4,870,413 cases at
+    depth 58 (whole seeds there reach 10^8 cases, so windows stay bounded)."""
     _say("sym 9x9 seed slices")
-    whole, _ = gpu.sym_fold(9, 58, 0, 0, seed_start=1000, seed_count=40)
-    parts = sum(gpu.sym_fold(9, 58, 0, 0, seed_start=s, seed_count=10)[0] for s in (1000, 1010, 1020, 1030))
+    whole, _ = gpu.sym_fold(9, 58, 0, 3000, seed_start=1001, seed_count=1)
+    parts = sum(gpu.sym_fold(9, 58, k, 1000, seed_start=1001, seed_count=1)[0] for k in (0, 1000, 2000))
     assert whole == parts
-    counts = gpu.sym_seed_case_counts(9, 58, 1000, 40)
-    a, _ = gpu.sym_fold(9, 58, int(counts[:7].sum()), int(counts[7:19].sum()), seed_start=1000, seed_count=40)
-    b, _ = gpu.sym_fold(9, 58, 0, 0, seed_start=1007, seed_count=12)
+    c0 = int(gpu.sym_seed_case_counts(9, 58, 1001, 1)[0])
+    assert c0 == 4870413
+    a, _ = gpu.sym_fold(9, 58, c0 - 200, 500, seed_start=1001, seed_count=2)
+    b = (gpu.sym_fold(9, 58, c0 - 200, 500, seed_start=1001, seed_count=1)[0] +
+         gpu.sym_fold(9, 58, 0, 300, seed_start=1002, seed_count=1)[0])
     assert a == b
 
 

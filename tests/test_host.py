@@ -217,3 +217,24 @@ def test_cli_sym_fold9_host_side_matches_reference(run):
     assert r.stdout == run["stdout"]
     if run["rc"] == 0:
         assert _clean(r.stderr) == run["stderr"]
+
+
+@pytest.mark.parametrize("order,lock", [("reverse", None), ("opposite", None), ("axes_first", "5"),
+                                        ("corners_first", None), ("corners_first", "1")])
+def test_sym_cell_orders_match_oracle(mf, oracle, monkeypatch, order, lock):
+    """MF_ORDER / MF_LOCK_CENTER_N renumber odd squares (sym_fold9.cpp:160-229):
+    product seeds and case windows equal the oracle's under each numbering
+    (7x7 with its 3x3 center, and 9x9 with the 5x5 center whose symmetry maps
+    follow the renumbered center)."""
+    monkeypatch.setenv("MF_ORDER", order)
+    if lock:
+        monkeypatch.setenv("MF_LOCK_CENTER_N", lock)
+    cases = [(7, 26, 3000, 400)] + ([(9, 62, 1000, 300)] if order == "reverse" else [])
+    for n, e, start, count in cases:
+        c1, m1, r1 = mf.sym_seeds(n)
+        c2, m2, r2 = oracle.sym9_seeds(n)
+        assert r1 == r2 and np.array_equal(c1[:, :c2.shape[1]] if n < 9 else c1, c2[:, :c1.shape[1]])
+        assert np.array_equal(m1, m2)
+        k1, w1 = mf.sym_cases(n, e, start, count)
+        k2, w2 = oracle.sym9_cases(n, e, start, count)
+        assert np.array_equal(k1, k2[:, :k1.shape[1]]) and np.array_equal(w1, w2)

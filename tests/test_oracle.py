@@ -242,3 +242,43 @@ def test_sym9_center_seeds(oracle, sym9_golden):
 def test_sym9_windows(oracle, sym9_golden, idx):
     w = sym9_golden["windows"][idx]
     assert oracle.sym9_answer(w["n"], w["expanded"], w["start"], w["max"])[0] == w["answer"]
+
+
+# MF_ORDER / MF_LOCK_CENTER_N (sym_fold9.cpp:160-229): ring-wise renumberings of odd squares.
+
+def _order_runs():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "sym9_cli.json")) as f:
+        runs = json.load(f)["runs"]
+    # The 7x7 depth-24 corners_first window (about 40 s here) is left to the
+    # GPU test of the command-line tool; a depth-28 window covers that order.
+    return [r for r in runs if "MF_ORDER" in r["env"] and r["env"].get("MF_STREAM_CASES") and r["rc"] == 0
+            and len(r["args"]) == 4 and "MF_SETUP_ONLY" not in r["env"]
+            and not (r["args"][0] == "7" and r["env"]["MF_EXPANDED_N"] == "24")]
+
+
+@pytest.mark.parametrize("run", _order_runs(), ids=lambda r: " ".join(f"{k}={v}" for k, v in sorted(r["env"].items())))
+def test_sym9_cell_orders_match_reference(oracle, monkeypatch, run):
+    """Stream windows under each renumbering equal the reference binary's answers."""
+    for k, v in run["env"].items():
+        monkeypatch.setenv(k, v)
+    n, mx, start = int(run["args"][0]), int(run["args"][2]), int(run["args"][3])
+    ans, _ = oracle.sym9_answer(n, int(run["env"]["MF_EXPANDED_N"]), start, mx)
+    assert ans == int(run["stdout"])
+
+
+@pytest.mark.parametrize("order", ["reverse", "opposite", "axes_first", "corners_first"])
+def test_sym9_cell_order_keeps_totals(oracle, monkeypatch, order):
+    """A renumbering changes the cases, not what they count: full 5x5 and 7x7
+    totals (7x7 through its 3x3-center seeds at depth 9, all cases) are the
+    published ones, and the numbering is a permutation of the cells."""
+    monkeypatch.setenv("MF_ORDER", order)
+    import ctypes as C
+    cyc = np.zeros(128, dtype=np.uint8)
+    cyc[:3] = [1, 2, 0]
+    cnt = (C.c_uint64 * 2)()
+    assert oracle.lib().orc9_count_case(5, cyc.ctypes.data_as(C.POINTER(C.c_uint8)), 3, cnt, None) == 0
+    assert 2 * 25 * (cnt[0] | (cnt[1] << 64)) == 186086600
+    seeds, mult, raw = oracle.sym9_seeds(7, 3)
+    assert (raw, len(seeds)) == (76, 11)
