@@ -10,6 +10,7 @@
  * Reference interfaces replaced (johnowhitaker/map_folding, file:line):
  *   mf_gpu_folding        gpu_folding.cu:209-311  (main: frontier -> launch -> total)
  *   mf_lunnon_frontier    gpu_folding_v2.cu:220-256, mf_fast.cpp:185-221 (collect_states)
+ *   mf_lunnon_check_states  mf.c:82-142 (the insertions a state must replay)
  *   mf_lunnon_solve       gpu_folding_v2.cu:381-453 (solve_kernel / solve_state_device),
  *                         gpu_folding.cu:123-207 (compute_counts)
  *   mf_sym_seeds          sym_fold.cpp:689-710 (recurse_generate + deduplicate_cases)
@@ -118,6 +119,15 @@ typedef struct {
  * and MF_ERR_CAPACITY is returned).  out_b may be NULL to count only. */
 int mf_lunnon_frontier(const int32_t *dims, int32_t ndim, int32_t depth,
                        uint8_t *out_b, int64_t max_states, int64_t *num_states);
+
+/* Check caller-supplied states (no GPU needed): each must be a state the
+ * folding DFS reaches (mf.c with its normal-form filter), i.e. leaves
+ * depth, depth-1, ..., 2 each sit directly below a position the gap walks of
+ * the smaller state allow, never below the root.  Every Lunnon entry point
+ * that takes states applies this check; on failure *first_bad is the index
+ * of the first rejected state. */
+int mf_lunnon_check_states(const int32_t *dims, int32_t ndim, int32_t depth, const uint8_t *states_b,
+                           int64_t num, int64_t *first_bad);
 
 /* Resume `num` frontier states on the GPU and count all completions under
  * each (solve_state_device semantics: stop at the state's own level).

@@ -98,6 +98,7 @@ def lib() -> C.CDLL:
         "mf_gpu_folding": (C.c_int, [P(i32), i32, i32, P(MFSolveOptions), P(MFResult), P(i64)]),
         "mf_lunnon_batch": (C.c_int, [P(i32), P(i32), i32, P(MFSolveOptions), P(u64), P(MFResult)]),
         "mf_sym_seeds": (C.c_int, [i32, P(u8), P(u64), i64, P(i64), P(i64)]),
+        "mf_lunnon_check_states": (C.c_int, [P(i32), i32, i32, P(u8), i64, P(i64)]),
         "mf_sym_cases": (C.c_int, [i32, i32, u64, u64, P(u8), P(u64), P(i64)]),
         "mf_sym_case_total": (C.c_int, [i32, i32, P(u64)]),
         "mf_sym_cases_slice": (C.c_int, [i32, i32, u64, u64, u64, u64, P(u8), P(u64), P(i64)]),

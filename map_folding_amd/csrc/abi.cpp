@@ -536,12 +536,6 @@ bool valid_cycle(const uint8_t *cycle, int level) {
     return c == 0 && seen == ((static_cast<u128>(1) << level) - 1);
 }
 
-// A Lunnon state must be one circular list through leaves [0, depth] with
-// leaf 1 directly below the root (the normal form of the frontier).
-bool valid_list(const uint8_t *b, int depth) {
-    return depth >= 1 && b[0] == 1 && valid_cycle(b, depth + 1);
-}
-
 WorkItem sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
     WorkItem it;
     std::memset(&it, 0, sizeof(it));
@@ -665,6 +659,23 @@ int mf_lunnon_frontier(const int32_t *dims, int32_t ndim, int32_t depth, uint8_t
     return MF_OK;
 }
 
+int mf_lunnon_check_states(const int32_t *dims, int32_t ndim, int32_t depth, const uint8_t *states_b, int64_t num,
+                           int64_t *first_bad) {
+    if (first_bad) *first_bad = -1;
+    if (num < 0 || (num > 0 && !states_b)) return fail(MF_ERR_ARG, "bad state arguments");
+    LunnonProblem pr;
+    std::string err;
+    if (!dims || !lunnon_problem(dims, ndim, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    if (depth < 1 || depth >= pr.n) return fail(MF_ERR_DEPTH, "depth must be between 1 and %d", pr.n - 1);
+    for (int64_t s = 0; s < num; ++s)
+        if (!lunnon_reachable(pr, states_b + s * (depth + 1), depth)) {
+            if (first_bad) *first_bad = s;
+            return fail(MF_ERR_ARG, "state %lld is not a folding-DFS state over leaves 0..%d",
+                        static_cast<long long>(s), depth);
+        }
+    return MF_OK;
+}
+
 int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth, const uint8_t *states_b, int64_t num,
                               mf_frontier **out) {
     int rc = require_init();
@@ -675,10 +686,7 @@ int mf_lunnon_frontier_create(const int32_t *dims, int32_t ndim, int32_t depth, 
     if (!dims || !lunnon_problem(dims, ndim, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
     if (depth < 1 || depth >= pr.n) return fail(MF_ERR_DEPTH, "depth must be between 1 and %d", pr.n - 1);
     if (num > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many states for one frontier");
-    for (int64_t s = 0; s < num; ++s)
-        if (!valid_list(states_b + s * (depth + 1), depth))
-            return fail(MF_ERR_ARG, "state %lld is not a normal-form list over leaves 0..%d", static_cast<long long>(s),
-                        depth);
+    if ((rc = mf_lunnon_check_states(dims, ndim, depth, states_b, num, nullptr)) != MF_OK) return rc;
     auto *f = new mf_frontier();
     f->kind = 2;
     f->maps.push_back(pr);

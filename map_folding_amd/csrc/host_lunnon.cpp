@@ -152,6 +152,44 @@ class FrontierWalker {
 
 }  // namespace
 
+bool lunnon_reachable(const LunnonProblem &pr, const uint8_t *state, int depth) {
+    if (depth < 1 || depth >= pr.n || state[0] != 1) return false;
+    uint8_t b[kMaxLeaves + 2];
+    std::memcpy(b, state, static_cast<size_t>(depth) + 1);
+    // One circular list through exactly the leaves 0..depth.
+    uint64_t seen = 0;
+    int c = 0;
+    for (int k = 0; k <= depth; ++k) {
+        if (b[c] > depth) return false;
+        seen |= 1ull << c;
+        c = b[c];
+    }
+    if (c != 0 || seen != (depth >= 63 ? ~0ull : (2ull << depth) - 1)) return false;
+    for (int l = depth; l >= 2; --l) {
+        int m = 0;   // the leaf directly above l
+        while (b[m] != l) ++m;
+        if (m == 0) return false;   // below the root: the normal form never inserts there
+        b[m] = b[l];
+        // Gap walks of leaf l in the list of leaves 0..l-1 (push_gaps above),
+        // each bounded by l steps.
+        uint64_t meet = ~0ull;
+        for (int i = 0; i < pr.ndim; ++i) {
+            int x = pr.at(i, l, l);
+            if (x == l) continue;
+            uint64_t reach = 0;
+            int steps = 0;
+            do {
+                reach |= 1ull << x;
+                x = pr.at(i, l, b[x]);
+                if (++steps > l) return false;
+            } while (x != l);
+            meet &= reach;
+        }
+        if (((meet >> m) & 1ull) == 0) return false;
+    }
+    return true;
+}
+
 int64_t lunnon_frontier(const LunnonProblem &pr, int depth, uint8_t *out, int64_t max_states) {
     FrontierWalker w(pr, depth);
     int64_t count = 0;

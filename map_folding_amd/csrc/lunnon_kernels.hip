@@ -50,7 +50,6 @@ struct LWork {
     bool have;
     bool popping;     // backtracking continues this step (no gap computation)
     bool progressed;  // a counted gap computation of this item has completed
-    bool bad;         // a walk exceeded l steps: the state was not a folding state
 };
 
 struct LTotals {
@@ -69,9 +68,14 @@ __device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m,
 }
 
 // Legal positions for leaf l: mask of m < l reached by the walk of every
-// dimension with d[i][l][l] != l (all m < l if none constrains l).  A walk
-// visits each position at most once, so it ends within l steps; the guard only
-// trips on a state that no folding DFS can produce.
+// dimension with d[i][l][l] != l (all m < l if none constrains l).  Every state
+// here is one the folding DFS reaches (inputs are checked on the host by
+// replaying their insertions, lunnon_reachable; split pieces are states of
+// this DFS), and on those each walk returns to l within l steps, as in mf.c.
+// The loop carries no step guard.  A guard's second loop exit made the
+// compiler track each lane's exit reason: 22 scalar and branch instructions
+// per step against 5 now (8 and 6 VALU), and the 6x6 depth-18 frontier ran in
+// 217 ms against 142 ms without it -- the walk was bound by scalar issue.
 __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
                                              uint32_t &steps, uint32_t &cands) {
     const uint32_t l = w.l;
@@ -86,15 +90,10 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         if (m == l) continue;
         constrained = true;
         uint64_t reach = 0;
-        uint32_t guard = 0;
         do {
             ++steps;
             reach |= 1ull << m;
             m = lds[row + lds[row_at(ln.cl, m)]];
-            if (++guard > l) {
-                w.bad = true;
-                break;
-            }
         } while (m != l);
         meet &= reach;
         reach_any |= reach;
@@ -115,7 +114,6 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
     if (!w.popping) {
         uint32_t ns = 0, nc = 0;
         uint64_t g = gap_mask(lds, ln, w, ns, nc);
-        if (w.bad) return true;
         if (w.floor == 0) {
             w.progressed = true;
             w.inodes += static_cast<uint32_t>(__popcll(g));
@@ -195,7 +193,6 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkIte
     w.have = true;
     w.popping = false;
     w.progressed = false;
-    w.bad = false;
     if (mask == 0) {
         w.l = level;
         w.floor = it->mult >> 8;
@@ -208,7 +205,6 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkIte
 }
 
 __device__ __forceinline__ void lretire(const LunnonArgs &a, const LWork &w, LTotals &t) {
-    if (w.bad) atomicOr(a.acc + ACC_ERROR, 2ull);
     const unsigned long long v = static_cast<unsigned long long>(w.n) * w.comps;
     t.sum += v;
     t.nodes += w.inodes;

@@ -33,6 +33,13 @@ struct LunnonProblem {
 // Returns an empty optional-like flag through `ok`; error text in `err`.
 bool lunnon_problem(const int32_t *dims, int ndim, LunnonProblem &pr, std::string &err);
 
+// True if the folding DFS (mf.c:82-142, with the normal-form filter) reaches
+// this state: leaves depth, depth-1, ..., 2 each sit directly below a position
+// that the gap walks of the smaller state allow, never below the root.  The
+// DFS kernels rely on it: every gap walk of a reachable state and of its
+// descendants returns to its leaf, so they carry no step guard.
+bool lunnon_reachable(const LunnonProblem &pr, const uint8_t *state, int depth);
+
 // Canonical frontier (collect_states order).  Appends b[0..D] rows to `out`
 // until `max_states` rows were written; returns the total frontier size.
 int64_t lunnon_frontier(const LunnonProblem &pr, int depth, uint8_t *out, int64_t max_states);

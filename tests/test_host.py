@@ -238,3 +238,30 @@ def test_sym_cell_orders_match_oracle(mf, oracle, monkeypatch, order, lock):
         k1, w1 = mf.sym_cases(n, e, start, count)
         k2, w2 = oracle.sym9_cases(n, e, start, count)
         assert np.array_equal(k1, k2[:, :k1.shape[1]]) and np.array_equal(w1, w2)
+
+
+@pytest.mark.parametrize("dims,depth", [([2, 3], 4), ([3, 3], 6), ([2, 4], 6), ([3, 4], 7), ([2, 2, 2], 6)])
+def test_lunnon_reachable_states_are_the_frontier(mf, dims, depth):
+    """mf_lunnon_check_states accepts exactly the states the folding DFS
+    reaches: over every circular list of leaves 0..depth with leaf 1 below
+    the root, the accepted ones are the frontier at that depth (the GPU
+    kernels' gap walks rely on this: they carry no step guard)."""
+    import ctypes as C
+    import itertools
+    L = mf._abi.lib()
+    d = np.asarray(dims, dtype=np.int32)
+    front = {bytes(r) for r in mf.lunnon_frontier(dims, depth)}
+    accepted = set()
+    for perm in itertools.permutations(range(2, depth + 1)):
+        order = [0, 1, *perm]                      # stack order from the root
+        b = np.zeros(depth + 1, dtype=np.uint8)
+        for k, leaf in enumerate(order):
+            b[leaf] = order[(k + 1) % len(order)]
+        bad = C.c_int64()
+        rc = L.mf_lunnon_check_states(d.ctypes.data_as(C.POINTER(C.c_int32)), len(dims), depth,
+                                      b.ctypes.data_as(C.POINTER(C.c_uint8)), 1, C.byref(bad))
+        if rc == 0:
+            accepted.add(bytes(b))
+        else:
+            assert bad.value == 0
+    assert accepted == front
