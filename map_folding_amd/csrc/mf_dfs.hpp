@@ -87,6 +87,7 @@ struct WideStack {
         *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = e;
     }
     static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
+    static __device__ __forceinline__ uint64_t encode(uint64_t g, uint32_t) { return g; }
     static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
     // 0 = no sibling after the current child, 1 = advance() gives it.
     static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
@@ -125,6 +126,13 @@ struct CompactStackT {
         const uint32_t j2 = mask_ctz(r);
         const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
         return j | (j2 << B) | (state << (2 * B));
+    }
+    // The same with the number of children (popcount of g) already known.
+    static __device__ __forceinline__ uint32_t encode(Mask g, uint32_t count) {
+        const uint32_t j = mask_ctz(g);
+        if (count == 1) return j;
+        const uint32_t j2 = mask_ctz(g & (g - 1));
+        return j | (j2 << B) | ((count > 2 ? 2u : 1u) << (2 * B));
     }
     static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & kField; }
     // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.

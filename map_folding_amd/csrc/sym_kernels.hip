@@ -49,8 +49,9 @@ namespace {
 // Cell geometry of a launch: up to 64 cells (u64 masks, 80-byte items) or up
 // to 96 (9x9: 81 cells, u128 masks, 128-byte items).  LDS: a partner word per
 // cell (byte p = partner on page p, n2 = none), the same page-major (byte
-// kPageOff + kCells*p + c), then per wave a kCells-entry byte row per lane and
-// the stack.
+// kPageOff + kCells*p + c), a byte per cell c with bit p set when c's page-p
+// partner is below c (the pages get_good walks for c), then per wave a
+// kCells-entry byte row per lane and the stack.
 template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_>
 struct Geometry {
     static constexpr uint32_t kCells = kCells_;
@@ -59,12 +60,13 @@ struct Geometry {
     using Stack = Stack_;
     static constexpr uint32_t kPartnerOff = 0;
     static constexpr uint32_t kPageOff = 4 * kCells;
-    static constexpr uint32_t kTableBytes = 8 * kCells;
+    static constexpr uint32_t kActOff = 8 * kCells;   // per cell i: pages whose partner of i is below i
+    static constexpr uint32_t kTableBytes = 9 * kCells;
     static constexpr uint32_t kCycleBytes = kCells * 64;       // one row of kCells bytes per lane
     static constexpr uint32_t kStateBytes = kCells;
-    static __device__ __forceinline__ Mask below(uint32_t i) {
-        return i >= 8 * sizeof(Mask) ? ~static_cast<Mask>(0) : ((static_cast<Mask>(1) << i) - 1);
-    }
+    // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
+    // j < i, so it stays under the mask width (64 up to 8x8, 128 for 9x9).
+    static __device__ __forceinline__ Mask below(uint32_t i) { return (static_cast<Mask>(1) << i) - 1; }
     static __device__ __forceinline__ Mask item_mask(const WorkItem *it) { return it->mask; }
     static __device__ __forceinline__ Mask item_mask(const WorkItem128 *it) {
         return (static_cast<Mask>(it->mask_hi) << 64) | it->mask_lo;
@@ -163,9 +165,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
     Mask good = G::below(i);
     const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + G::kPartnerOff)[i];
     if (MF_SYM_WALK_ROUNDS) {
-        uint32_t act = 0;   // pages whose partner of cell i is already placed
-#pragma unroll
-        for (int p = 0; p < 4; ++p) act |= (((ew >> (8 * p)) & 0xffu) < i ? 1u : 0u) << p;
+        uint32_t act = lds[G::kActOff + i];   // pages whose partner of cell i is already placed
         while (act != 0 && good != 0) {
             const uint32_t p = static_cast<uint32_t>(__builtin_ctz(act));
             act &= act - 1u;
@@ -202,22 +202,25 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
         typename G::Mask g = good_mask<G>(lds, ln.cl, w.i, nw, ns);
+        uint32_t cnt;   // children of this node
         if (w.floor == 0) {
             ++w.inodes;
             if (EV) {
                 w.iwalks += nw;
                 w.isteps += ns;
             }
+            cnt = mask_popcount(g);
             if (w.i == last) {
-                w.leaves += mask_popcount(g);
+                w.leaves += cnt;
                 g = 0;
             }
         } else {
             g &= ~G::below(w.floor);
             w.floor = 0;
+            cnt = mask_popcount(g);
         }
         if (g != 0) {
-            const typename Stack::Entry e = Stack::encode(g);
+            const typename Stack::Entry e = Stack::encode(g, cnt);
             Stack::put(lds, ln, w.i, e);
             place(lds, ln, Stack::child(e), w.i);
             ++w.i;
@@ -341,8 +344,13 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
     if (tid < G::kCells) {
         const uint32_t w = a.edge_word[tid];
         reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = w;
+        uint32_t act = 0;
 #pragma unroll
-        for (uint32_t p = 0; p < 4; ++p) lds[G::kPageOff + G::kCells * p + tid] = static_cast<uint8_t>(w >> (8u * p));
+        for (uint32_t p = 0; p < 4; ++p) {
+            lds[G::kPageOff + G::kCells * p + tid] = static_cast<uint8_t>(w >> (8u * p));
+            act |= (((w >> (8u * p)) & 0xffu) < tid ? 1u : 0u) << p;
+        }
+        lds[G::kActOff + tid] = static_cast<uint8_t>(act);
     }
     __syncthreads();
 }
