@@ -118,6 +118,16 @@ def run_lunnon(args, mf, rl):
         want = golden["square_totals"]["6"]
         label = f"6x6 Lunnon DFS, partition depth 18 ({states.shape[0]} states)"
         items = states.shape[0]
+        if not args.no_tune and not (args.budget or args.tail or args.waves_per_cu):
+            from map_folding_amd import autotune
+            t1 = time.time()
+            cfg = autotune.tune_resplit(front, budgets=(32, 64), tail_mult=(1,), waves=(16, 32), log=log)
+            batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=cfg["budget"],
+                                      tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"], core=args.core)
+            tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "kernel_ms")}
+            tuned.update({"trials": len(cfg["trials"]), "tune_s": time.time() - t1})
+            log(f"autotune: budget {cfg['budget']} tail {cfg['tail']} waves {cfg['waves_per_cu']} "
+                f"({cfg['kernel_ms']:.1f} ms) in {tuned['tune_s']:.1f}s")
     else:
         shapes = rect_shapes(args.seed)
         depths = [rect_depth(p, q) for p, q in shapes]
