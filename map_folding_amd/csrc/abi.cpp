@@ -22,9 +22,9 @@
 
 namespace mf {
 hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid, uint32_t block, uint32_t lds,
-                      bool events, bool cells96, hipStream_t);
-uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, bool cells96);
-const void *sym_kernel_ptr(int core, bool persistent, bool events, bool cells96);
+                      bool events, uint32_t cells, hipStream_t);
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells);
+const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells);
 hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t grid, uint32_t block,
                          uint32_t lds, bool events, hipStream_t);
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block);
@@ -477,11 +477,12 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         SymArgs a;
         uint32_t levels = 0;
         sym_args(*f, a, &levels);
-        const uint32_t lds = sym_lds_bytes(core, levels, t.block, c96);
+        const uint32_t cells = static_cast<uint32_t>(f->sym.n2);
+        const uint32_t lds = sym_lds_bytes(core, levels, t.block, cells);
         auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
-            return launch_sym(args, core, persistent, grid, t.block, lds, t.events, c96, g.stream);
+            return launch_sym(args, core, persistent, grid, t.block, lds, t.events, cells, g.stream);
         };
-        const void *ks = sym_kernel_ptr(core, false, t.events, c96), *kp = sym_kernel_ptr(core, true, t.events, c96);
+        const void *ks = sym_kernel_ptr(core, false, t.events, cells), *kp = sym_kernel_ptr(core, true, t.events, cells);
         const uint64_t origins = f->origins ? f->origins : f->num;
         if (c96) {
             const WorkItem128 *dev = static_cast<const WorkItem128 *>(f->items.ptr);

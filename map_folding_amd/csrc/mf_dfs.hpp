@@ -26,8 +26,9 @@ __device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) { ret
 __device__ __forceinline__ uint32_t row_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
 
 // Copy a work item's first kBytes state bytes (a multiple of 16) into the
-// lane's row, and back.
-template <uint32_t kBytes = 64>
+// lane's row, and back.  The row holds kRow entries (kRow <= kBytes); state
+// bytes past it are zero in every item (cells past n*n) and are not stored.
+template <uint32_t kBytes = 64, uint32_t kRow = kBytes>
 __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
 #pragma unroll
@@ -35,18 +36,21 @@ __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_
         const uint4 v = s4[q];
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int k = 0; k < 16; ++k) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+        for (uint32_t k = 0; k < 16; ++k)
+            if (16u * q + k < kRow) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
     }
 }
 
-template <uint32_t kBytes = 64>
+template <uint32_t kBytes = 64, uint32_t kRow = kBytes>
 __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
     uint4 *d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
     for (uint32_t q = 0; q < kBytes / 16; ++q) {
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < 16; ++k) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+        for (uint32_t k = 0; k < 16; ++k)
+            if (16u * q + k < kRow)
+                w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
         d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }

@@ -52,9 +52,10 @@ namespace {
 // kPageOff + kCells*p + c), a byte per cell c with bit p set when c's page-p
 // partner is below c (the pages get_good walks for c), then per wave a
 // kCells-entry byte row per lane and the stack.
-template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_>
+template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_>
 struct Geometry {
     static constexpr uint32_t kCells = kCells_;
+    static constexpr uint32_t kRowCells = kRowCells_;   // entries of a lane's LDS row (cells of the largest map)
     using Mask = Mask_;
     using Item = Item_;
     using Stack = Stack_;
@@ -62,7 +63,7 @@ struct Geometry {
     static constexpr uint32_t kPageOff = 4 * kCells;
     static constexpr uint32_t kActOff = 8 * kCells;   // per cell i: pages whose partner of i is below i
     static constexpr uint32_t kTableBytes = 9 * kCells;
-    static constexpr uint32_t kCycleBytes = kCells * 64;       // one row of kCells bytes per lane
+    static constexpr uint32_t kCycleBytes = kRowCells * 64;    // one row of kRowCells bytes per lane
     static constexpr uint32_t kStateBytes = kCells;
     // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
     // j < i, so it stays under the mask width (64 up to 8x8, 128 for 9x9).
@@ -80,7 +81,20 @@ struct Geometry {
 
 using G64 = Geometry<64, uint64_t, WorkItem, CompactStack>;
 using G64Wide = Geometry<64, uint64_t, WorkItem, WideStack>;
-using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128>;
+// 9x9: 81 cells in LDS rows (5,184 B per wave instead of 6,144), so four
+// 4-wave blocks fit a CU at depth 44 instead of three.
+using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128, 81>;
+// Up to 7x7: 49-entry rows (3,136 B per wave): six 4-wave blocks per CU at
+// 7x7 depth 24 instead of five.
+using G49 = Geometry<64, uint64_t, WorkItem, CompactStack, 49>;
+
+// The geometry of a launch: `cells` = n*n of the map.
+enum class Geo { k49, k64, k64Wide, k96 };
+inline Geo geometry(int core, uint32_t cells) {
+    if (cells > 64) return Geo::k96;
+    if (core == 1) return Geo::k64Wide;
+    return cells <= 49 ? Geo::k49 : Geo::k64;
+}
 
 __device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_at(lane_base, cell); }
 
@@ -263,7 +277,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
 template <typename G>
 __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln,
                                                  const typename G::Item *__restrict__ it, Work &w) {
-    row_load<G::kStateBytes>(lds, ln.cl, it->state);
+    row_load<G::kStateBytes, G::kRowCells>(lds, ln.cl, it->state);
     w.origin = it->origin;
     w.mult = it->mult;
     w.root = it->level;
@@ -377,7 +391,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         if (emit) {
             if (slot < a.spill_cap) {
                 Item *dst = static_cast<Item *>(a.spill) + slot;
-                row_store<G::kStateBytes>(lds, ln.cl, dst->state);
+                row_store<G::kStateBytes, G::kRowCells>(lds, ln.cl, dst->state);
                 G::set_item_mask(dst, mask);
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.mult);
@@ -528,11 +542,15 @@ const void *pick(bool persistent, bool events) {
 }
 }  // namespace
 
-// core: 1 = wide u64 stack, otherwise the compact 16-bit stack; cells96: the
-// 9x9 geometry (compact stack only).
-const void *sym_kernel_ptr(int core, bool persistent, bool events, bool cells96) {
-    if (cells96) return pick<G96>(persistent, events);
-    return core == 1 ? pick<G64Wide>(persistent, events) : pick<G64>(persistent, events);
+// core: 1 = wide u64 stack (up to 8x8), otherwise the compact 16-bit stack;
+// cells = n*n picks the row size (49, 64 or, for 9x9, 81 entries).
+const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells) {
+    switch (geometry(core, cells)) {
+        case Geo::k96: return pick<G96>(persistent, events);
+        case Geo::k64Wide: return pick<G64Wide>(persistent, events);
+        case Geo::k49: return pick<G49>(persistent, events);
+        default: return pick<G64>(persistent, events);
+    }
 }
 
 // Between graph rounds: rewind the claim cursor and empty the next spill
@@ -550,16 +568,26 @@ hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t 
 }
 
 hipError_t launch_sym(const SymArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,
-                      uint32_t lds_bytes, bool events, bool cells96, hipStream_t s) {
+                      uint32_t lds_bytes, bool events, uint32_t cells, hipStream_t s) {
     void *args[] = {const_cast<SymArgs *>(&a)};
-    return hipLaunchKernel(sym_kernel_ptr(core, persistent, events, cells96), dim3(grid), dim3(block), args,
+    return hipLaunchKernel(sym_kernel_ptr(core, persistent, events, cells), dim3(grid), dim3(block), args,
                            lds_bytes, s);
 }
 
-uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, bool cells96) {
-    if (cells96) return G96::kTableBytes + (block / 64u) * (G96::kCycleBytes + levels * G96::Stack::kSlotBytes);
-    const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
-    return G64::kTableBytes + (block / 64u) * (G64::kCycleBytes + levels * slot);
+namespace {
+template <typename G>
+uint32_t lds_of(uint32_t levels, uint32_t block) {
+    return G::kTableBytes + (block / 64u) * (G::kCycleBytes + levels * G::Stack::kSlotBytes);
+}
+}  // namespace
+
+uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells) {
+    switch (geometry(core, cells)) {
+        case Geo::k96: return lds_of<G96>(levels, block);
+        case Geo::k64Wide: return lds_of<G64Wide>(levels, block);
+        case Geo::k49: return lds_of<G49>(levels, block);
+        default: return lds_of<G64>(levels, block);
+    }
 }
 
 }  // namespace mf
