@@ -27,7 +27,7 @@ uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells);
 hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t grid, uint32_t block,
                          uint32_t lds, bool events, hipStream_t);
-uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block);
+uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows);
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 }  // namespace mf
@@ -420,6 +420,10 @@ int lunnon_args(const mf_frontier &f, LunnonArgs &a, std::vector<uint8_t> &blob,
     }
     a.num_maps = static_cast<uint32_t>(f.maps.size());
     a.tables_bytes = static_cast<uint32_t>(blob.size());
+    // Leaves 0..n-1 of the largest map: leaf n is never linked (inserting it
+    // counts completions), so a row needs n entries.
+    a.row_entries = 0;
+    for (const LunnonProblem &pr : f.maps) a.row_entries = std::max<uint32_t>(a.row_entries, pr.n);
     a.levels = lv;
     *levels = lv;
     return MF_OK;
@@ -502,7 +506,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         HIP_TRY(hipMemcpyAsync(g.tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream));
         a.tables = static_cast<const uint8_t *>(g.tables.ptr);
         a.per_map = nullptr;
-        const uint32_t lds = lunnon_lds_bytes(t.core, a.tables_bytes, levels, t.block);
+        const uint32_t lds = lunnon_lds_bytes(t.core, a.tables_bytes, levels, t.block, a.row_entries);
         if (per_map) {
             if ((rc = g.per_map.ensure(f->maps.size() * 8)) != MF_OK) return rc;
             a.per_map = static_cast<unsigned long long *>(g.per_map.ptr);

@@ -171,8 +171,8 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
 // in the upper byte of `mult` (the low byte is n).
 template <typename Stack>
 __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkItem *__restrict__ it,
-                                           const LunnonMap *maps, LWork &w) {
-    row_load(lds, ln.cl, it->state);
+                                           const LunnonMap *maps, uint32_t rows, LWork &w) {
+    row_load_n(lds, ln.cl, it->state, rows);
     const LunnonMap mp = maps[it->map];
     w.n = mp.n;
     w.ndim = mp.ndim;
@@ -232,16 +232,17 @@ __device__ __forceinline__ void lflush(const LunnonArgs &a, uint32_t lane, const
     }
 }
 
-// LDS: [tables][maps][per wave: b rows | stack].
+// LDS: [tables][maps][per wave: b rows of row_entries entries | stack].
 __device__ __forceinline__ uint32_t tables_span(uint32_t bytes) { return (bytes + 15u) & ~15u; }
 
 template <typename Stack>
-__device__ __forceinline__ Lane lmake_lane(uint32_t tab_span, uint32_t levels) {
+__device__ __forceinline__ Lane lmake_lane(uint32_t tab_span, uint32_t levels, uint32_t rows) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t wbase = tab_span + kMapBytes + wave * (kRowBytes + levels * Stack::kSlotBytes);
+    const uint32_t row_bytes = rows * 64u;
+    const uint32_t wbase = tab_span + kMapBytes + wave * (row_bytes + levels * Stack::kSlotBytes);
     Lane ln;
     ln.cl = row_lane_base(wbase, lane);
-    ln.sl = wbase + kRowBytes + lane * Stack::kLaneBytes;
+    ln.sl = wbase + row_bytes + lane * Stack::kLaneBytes;
     ln.base = 0;
     return ln;
 }
@@ -277,7 +278,7 @@ __device__ __forceinline__ void lspill_all(const LunnonArgs &a, uint8_t *lds, co
         if (emit) {
             if (slot < a.spill_cap) {
                 WorkItem *dst = a.spill + slot;
-                row_store(lds, ln.cl, dst->state);
+                row_store_n(lds, ln.cl, dst->state, a.row_entries);
                 dst->mask = mask;
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.n | (floor << 8));
@@ -322,13 +323,13 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     const uint32_t span = tables_span(a.tables_bytes);
     lstage(a, lds, span);
     const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
-    Lane ln = lmake_lane<Stack>(span, a.levels);
+    Lane ln = lmake_lane<Stack>(span, a.levels, a.row_entries);
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     LTotals tot;
     LWork w;
     w.have = false;
     w.popping = false;
-    if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
+    if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, a.row_entries, w);
     while (w.have) {
         if (ldfs_step<Stack, EV>(lds, ln, w)) {
             lretire(a, w, tot);
@@ -356,7 +357,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     const uint32_t span = tables_span(a.tables_bytes);
     lstage(a, lds, span);
     const LunnonMap *maps = reinterpret_cast<const LunnonMap *>(lds + span);
-    Lane ln = lmake_lane<Stack>(span, a.levels);
+    Lane ln = lmake_lane<Stack>(span, a.levels, a.row_entries);
     const uint32_t lane = threadIdx.x & 63u;
     const unsigned long long below = (1ull << lane) - 1ull;
 
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                 if (got + want >= num_items) drained = true;
                 if (!w.have) {
                     const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
-                    if (idx < num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, w);
+                    if (idx < num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, a.row_entries, w);
                 }
             } else {
                 drained = __hip_atomic_load(a.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= num_items;
@@ -423,9 +424,9 @@ const void *lpick(bool persistent, bool events) {
 }
 }  // namespace
 
-uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block) {
+uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows) {
     const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
-    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (kRowBytes + levels * slot);
+    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (rows * 64u + levels * slot);
 }
 
 // core: 1 = wide u64 stack, otherwise the compact 16-bit stack.

@@ -13,8 +13,6 @@
 
 namespace mf {
 
-constexpr uint32_t kRowBytes = 4096;   // 64 entries x 64 lanes
-
 __device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ? ~0ull : ((1ull << i) - 1ull); }
 
 // Row layout, byte-interleaved: entry c of lane L at byte c*64 + L -- one VALU
@@ -51,6 +49,32 @@ __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8
         for (uint32_t k = 0; k < 16; ++k)
             if (16u * q + k < kRow)
                 w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+        d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// The same for a row of `rows` entries known only at run time (rows <= 64):
+// entries past it are zero in every item and are not stored.
+__device__ __forceinline__ void row_load_n(uint8_t *lds, uint32_t lb, const uint8_t *src, uint32_t rows) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint4 v = s4[q];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k)
+            if (16u * q + k < rows) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+    }
+}
+
+__device__ __forceinline__ void row_store_n(const uint8_t *lds, uint32_t lb, uint8_t *dst, uint32_t rows) {
+    uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k)
+            if (16u * q + k < rows) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
         d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
