@@ -76,6 +76,89 @@ def _traffic(args):
         return json.load(f)["hbm_bytes_per_launch"]
 
 
+# CPU baseline with the reference's own solver (built from its sources into
+# oracle/_ref by `make -C oracle ref`): a window of the same case stream, the
+# binary's threads = the host cores.  Windows are chosen for 10-30 s of CPU.
+REF_WINDOWS = {"sym8": ("sym_fold", 0, 12000), "sym7": ("sym_fold", 200000, 20000), "sym9": ("sym_fold9", 0, 2000)}
+
+
+def reference_cpu(args, mf, n, level, threads):
+    """Time the reference binary on a bounded window; its answer must equal the
+    GPU's count of the same window, whose exact node count gives the rate.
+    None if the binary is not there."""
+    import subprocess
+    exe_name, start, count = REF_WINDOWS[args.workload]
+    exe = os.path.join(ROOT, "oracle", "_ref", exe_name)
+    if not os.path.exists(exe):
+        return None
+    env = {k: v for k, v in os.environ.items() if not k.startswith("MF_")}
+    env.update({"MF_STREAM_CASES": "1", "MF_EXPANDED_N": str(level)})
+    t1 = time.perf_counter()
+    r = subprocess.run([exe, str(n), str(threads), str(count), str(start)], capture_output=True, text=True, env=env,
+                       timeout=600)
+    cs = time.perf_counter() - t1
+    if r.returncode != 0:
+        raise SystemExit(f"reference {exe_name} failed: {r.stderr[-300:]}")
+    answer = int(r.stdout.split()[0])
+    cyc, mul = mf.sym_cases(n, level, start, count)
+    res, _, _ = mf.sym_solve(n, level, cyc, mul, mf.SolveOptions(count_events=True))
+    if res.count * n * n != answer:
+        raise SystemExit(f"reference {exe_name} answered {answer}, the GPU {res.count * n * n}")
+    return {"value": res.nodes / cs, "unit": "nodes/s", "cores": threads, "kind": "reference",
+            "sample": f"oracle/_ref/{exe_name} (reference source built here) {n} {threads} {count} {start} with "
+                      f"MF_STREAM_CASES=1 MF_EXPANDED_N={level}: {count} cases ({res.nodes} nodes, answer {answer} "
+                      f"= the GPU's), {cs:.1f} s wall including its seed and stream setup"}
+
+
+def _run_ref(exe_name, argv):
+    """(stdout, seconds) of a reference binary in oracle/_ref, or None if it is
+    not built."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", exe_name)
+    if not os.path.exists(exe):
+        return None
+    t1 = time.perf_counter()
+    r = subprocess.run([exe] + [str(a) for a in argv], capture_output=True, text=True, timeout=600)
+    cs = time.perf_counter() - t1
+    if r.returncode != 0:
+        raise SystemExit(f"reference {exe_name} failed: {r.stderr[-300:]}")
+    return r.stdout, cs
+
+
+def reference_lunnon6(want, gpu_inserts):
+    """The reference's threaded solver on the whole 6x6 map from the same
+    depth-18 frontier (`mf_fast 6 18 <threads>`, one state per task).  It runs
+    the same DFS, so its inserts below the frontier are the GPU's exact count;
+    the frontier collection above it (112,024 states) is not counted."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    got = _run_ref("mf_fast", [6, 18, threads])
+    if got is None:
+        return None
+    out, cs = got
+    if int(out.split()[0]) != want:
+        raise SystemExit(f"reference mf_fast answered {out.split()[0]}, expected {want}")
+    return {"value": gpu_inserts / cs, "unit": "inserts/s", "cores": threads, "kind": "reference",
+            "sample": f"oracle/_ref/mf_fast 6 18 {threads} (reference source built here): the whole 6x6 map, "
+                      f"answer {want} = the GPU's, {gpu_inserts} inserts below the frontier, {cs:.1f} s wall"}
+
+
+def reference_rects(maps, answers, inserts):
+    """The reference's single-threaded mf.c on the maps of the port's sample
+    (`mf p q` each); the inserts are the port's exact count of the same DFS."""
+    total = 0.0
+    for (p, q), want in zip(maps, answers):
+        got = _run_ref("mf", [p, q])
+        if got is None:
+            return None
+        out, cs = got
+        if int(out.split()[0]) != want:
+            raise SystemExit(f"reference mf {p} {q} answered {out.split()[0]}, the oracle {want}")
+        total += cs
+    return {"value": inserts / total, "unit": "inserts/s", "cores": 1, "kind": "reference",
+            "sample": f"oracle/_ref/mf p q (reference source built here) on the maps {maps}, answers = the "
+                      f"oracle's, {inserts} inserts, {total:.1f} s wall"}
+
+
 def rect_shapes(seed, k=64):
     """The batched-instances draw: p uniform in {2,3,4}, then q uniform among
     the q with 12 <= p*q <= 30."""
@@ -95,6 +178,16 @@ def rect_depth(p, q):
     return max(2, min(p * q - 6, (2 * p * q) // 3))
 
 
+def roof_peak(rl, cus):
+    """The int32 VALU roof in T ops/s, measured on this device by valu_peak
+    (after the timed steps), with the spec-sheet figure beside it."""
+    ops, source = rl.measured_int32_peak(ROOT)
+    spec = rl.int32_peak_ops(cus or 256) / 1e12
+    log(f"int32 VALU peak: {ops:.4e} ops/s ({source}); CUs x 64 x 2.4 GHz = {spec*1e12:.4e}")
+    return ops / 1e12, {"peak_source": source, "peak_cus_x64_x2p4ghz": spec,
+                        "peak_published_fp32_lane_fma": rl.PUBLISHED_FP32_LANE_OPS / 1e12}
+
+
 def run_lunnon(args, mf, rl):
     """lunnon6: 6x6 map, frontier at depth 18 (112,024 states), resumed on the
     GPU; rect64: 64 seeded rectangles, each with its own frontier, all in one
@@ -104,7 +197,6 @@ def run_lunnon(args, mf, rl):
         golden = json.load(f)
     mf.init(0)
     cus = mf.device_cus()
-    peak = rl.int32_peak_ops(cus or 256) / 1e12
     batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=args.budget,
                               tail=args.tail, waves_per_cu=args.waves_per_cu, core=args.core)
     events = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=True, core=args.core)
@@ -175,6 +267,7 @@ def run_lunnon(args, mf, rl):
     ms = 1e3 * sum(wall) / len(wall)
     ops = rl.lunnon_ops(res.walks, res.steps, res.cands, res.nodes, res.leaves)
     achieved = ops / (sum(kern) / len(kern)) / 1e12
+    peak, peak_notes = roof_peak(rl, cus)
     plain = None
     if not args.no_plain and args.workload == "lunnon6":
         t1 = time.perf_counter()
@@ -184,7 +277,7 @@ def run_lunnon(args, mf, rl):
             raise SystemExit("static schedule disagreed")
         plain = {"nodes_per_s": res.nodes / pw, "ms": pw * 1e3, "speedup_batched_vs_plain": pw * 1e3 / ms}
         log(f"plain static: {pw*1e3:.1f} ms")
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu:
         from oracle import oracle
         t1 = time.perf_counter()
@@ -195,19 +288,29 @@ def run_lunnon(args, mf, rl):
         else:
             # Maps of the draw in order (those of at most 24 leaves, so that no
             # single map runs for minutes), until about 10 s of CPU work.
-            small, cst = [], {"inserts": 0}
+            small, answers, cst = [], [], {"inserts": 0}
             for p, q in shapes:
                 if p * q > 24:
                     continue
-                cst["inserts"] += oracle.lunnon_count([p, q])[1]["inserts"]
+                tot, st = oracle.lunnon_count([p, q])
+                cst["inserts"] += st["inserts"]
                 small.append((p, q))
+                answers.append(tot)
                 if time.perf_counter() - t1 > 10.0:
                     break
             sample = f"full counts of the draw's maps {small} (oracle C restatement of mf.c), 1 thread"
         cs = time.perf_counter() - t1
-        cpu = {"value": cst["inserts"] / cs, "unit": "inserts/s", "cores": 1, "kind": "port",
-               "sample": f"{sample}, {cs:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.3e} inserts/s")
+        cpu_port = {"value": cst["inserts"] / cs, "unit": "inserts/s", "cores": 1, "kind": "port",
+                    "sample": f"{sample}, {cs:.1f} s"}
+        log(f"cpu port: {cpu_port['value']:.3e} inserts/s")
+        if args.workload == "lunnon6":
+            cpu = reference_lunnon6(want, res.nodes)
+        else:
+            cpu = reference_rects(small, answers, cst["inserts"])
+        if cpu is not None:
+            log(f"cpu baseline (reference binary): {cpu['value']:.3e} inserts/s on {cpu['cores']} threads")
+        else:
+            cpu = cpu_port
     return {
         "metric": "DFS nodes/s & cases/s on 1 MI355X, batched+learned vs plain, bit-exact",
         "value": res.nodes / (ms / 1e3), "unit": "inserts/s", "n_gpus": 1, "steps": args.steps,
@@ -217,9 +320,9 @@ def run_lunnon(args, mf, rl):
         "autotune": tuned,
         "items_per_s": items / (ms / 1e3), "nodes": res.nodes, "gapcomps": res.walks, "walk_steps": res.steps,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TOP/s", "frac": achieved / peak,
-                     "traffic": None, "ops_per_step": ops,
+                     "traffic": None, "ops_per_step": ops, **peak_notes,
                      "ops_model": "10*gapcomps + 8*walk_steps + 6*candidates + 8*inserts + 2*completions"},
-        "plain": plain, "cpu_baseline": cpu, "frontier_gen_s": gen_s,
+        "plain": plain, "cpu_baseline": cpu, "cpu_port": cpu_port, "frontier_gen_s": gen_s,
     }
 
 
@@ -397,7 +500,7 @@ def main():
     kernel_s = sum(kern) / len(kern)
     ops = rl.sym_ops(gpu_nodes, walks, steps)
     achieved = ops / kernel_s / 1e12
-    peak = rl.int32_peak_ops(cus or 256) / 1e12
+    peak, peak_notes = roof_peak(rl, cus)
 
     # The plain static-frontier kernel (one lane per case, one launch) is
     # bounded by its single longest case; on the whole chunk it runs for
@@ -440,7 +543,7 @@ def main():
                 log(f"{name}: {len(pcs)} pieces, {qw*1e3:.1f} ms")
                 fr.close()
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu:
         from oracle import oracle
         m = min(args.cpu_cases, cyc.shape[0])
@@ -453,10 +556,15 @@ def main():
             _, _, csum, cst = oracle.sym9_solve(n, level, cyc[:m], mul[:m], threads=threads)
             src = "sym_fold9.cpp recurse_count (u128 masks)"
         cs = time.perf_counter() - t1
-        cpu = {"value": cst["nodes"] / cs, "unit": "nodes/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} cases of the chunk ({cst['nodes']} nodes), oracle C restatement of "
-                         f"{src} on {threads} threads, {cs:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.3e} nodes/s on {threads} threads ({cs:.1f}s)")
+        cpu_port = {"value": cst["nodes"] / cs, "unit": "nodes/s", "cores": threads, "kind": "port",
+                    "sample": f"first {m} cases of the chunk ({cst['nodes']} nodes), oracle C restatement of "
+                              f"{src} on {threads} threads, {cs:.1f} s"}
+        log(f"cpu port: {cpu_port['value']:.3e} nodes/s on {threads} threads ({cs:.1f}s)")
+        cpu = reference_cpu(args, mf, n, level, threads)
+        if cpu is not None:
+            log(f"cpu baseline (reference binary): {cpu['value']:.3e} nodes/s on {threads} threads")
+        else:
+            cpu = cpu_port
 
     out = {
         "metric": "DFS nodes/s & cases/s on 1 MI355X, batched+learned vs plain, bit-exact",
@@ -488,13 +596,14 @@ def main():
                      "frac": achieved / peak, "traffic": _traffic(args),
                      "ops_per_step": ops,
                      "ops_model": "16*nodes + 6*page_walks + 8*walk_steps (int32), GPU events of the timed step",
-                     "peak_model": "int32 VALU: CUs x 64 lanes x 2.4 GHz"},
+                     **peak_notes},
         "no_model": no_model,
         "learned": learned,
         "autotune": tuned,
         "plain": plain,
         "speedup_vs_plain": plain["speedup_batched_vs_plain"] if plain else None,
         "cpu_baseline": cpu,
+        "cpu_port": cpu_port,
         "frontier_gen_s": gen_s,
     }
     print(json.dumps(out), flush=True)
