@@ -48,9 +48,9 @@ namespace {
 
 // Cell geometry of a launch: up to 64 cells (u64 masks, 80-byte items) or up
 // to 96 (9x9: 81 cells, u128 masks, 128-byte items).  LDS: a partner word per
-// cell (byte p = partner on page p, n2 = none), the same page-major (byte
-// kPageOff + kCells*p + c), a byte per cell c with bit p set when c's page-p
-// partner is below c (the pages get_good walks for c), then per wave a
+// cell (bits 8p..8p+6 = partner on page p, n2 = none; bit 8p+7 set when that
+// partner is below the cell, i.e. get_good walks page p for it), the
+// partners page-major (byte kPageOff + kCells*p + c), then per wave a
 // kCells-entry byte row per lane and the stack.
 template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_>
 struct Geometry {
@@ -61,8 +61,7 @@ struct Geometry {
     using Stack = Stack_;
     static constexpr uint32_t kPartnerOff = 0;
     static constexpr uint32_t kPageOff = 4 * kCells;
-    static constexpr uint32_t kActOff = 8 * kCells;   // per cell i: pages whose partner of i is below i
-    static constexpr uint32_t kTableBytes = 9 * kCells;
+    static constexpr uint32_t kTableBytes = 8 * kCells;
     static constexpr uint32_t kCycleBytes = kRowCells * 64;    // one row of kRowCells bytes per lane
     static constexpr uint32_t kStateBytes = kCells;
     // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
@@ -130,19 +129,29 @@ struct Totals {
 };
 
 // get_good (sym_fold.cpp:179-204; u128 masks as sym_fold9.cpp:286-311) for
-// cell i against this lane's cycle.
-//
-// Walk rounds (MF_SYM_WALK_ROUNDS = 1, the default): lanes of a wave sit at
-// different levels, so they walk different subsets of the 4 pages (about 2
-// each).  With one loop per page index (MF_SYM_WALK_ROUNDS = 0) the wave runs
-// every page's loop; here round k walks each lane's k-th page with an earlier
-// neighbour (ascending, as before, so the walks and their early exit are
-// unchanged), and the wave runs max(#pages) rounds.  The page offset is then a
-// per-lane value: one more VALU op per walk step, and 12% less time on the
-// 8x8 depth-36 chunk.
-#ifndef MF_SYM_WALK_ROUNDS
-#define MF_SYM_WALK_ROUNDS 1
+// cell i against this lane's cycle.  Lanes of a wave sit at different levels,
+// so they walk different subsets of the 4 pages (about 2 each).  Walk modes:
+//   0  one loop per page index: the wave runs every page's loop;
+//   1  rounds: round k walks each lane's k-th page with an earlier neighbour
+//      (ascending, so the walks and their early exit are unchanged), and the
+//      wave runs max(#pages) rounds; the page offset becomes a per-lane
+//      value (12% less time than mode 0 on the 8x8 depth-36 chunk);
+//   2  pairs: a round walks two of the lane's pages in one loop.  Both
+//      chains step every iteration, a closed one in place (its loads stay
+//      on valid cells, its results are discarded), so that the two chains'
+//      LDS loads are in flight together: a walk step is two dependent LDS
+//      loads, and at 5 waves per SIMD their latency, not VALU issue, bounds
+//      the loop.  5% less time than mode 1 on the 8x8 chunk.
+// The default is pairs for u64 masks and rounds for u128 masks (9x9), where
+// two 128-bit bit-sets per iteration cost more than the overlap saves (5.6%
+// slower than rounds).  MF_SYM_WALK_MODE overrides it.
+#ifndef MF_SYM_WALK_MODE
+#define MF_SYM_WALK_MODE -1
 #endif
+template <typename G>
+constexpr int walk_mode() {
+    return MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE : (sizeof(typename G::Mask) > 8 ? 1 : 2);
+}
 
 // Where a page-p walk moves from cell j at level i: the page partner if it is
 // already placed, else j itself.  `prow` is the page's row of partner bytes.
@@ -177,20 +186,68 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                                                       uint32_t &walks, uint32_t &steps) {
     using Mask = typename G::Mask;
     Mask good = G::below(i);
+    // One load brings the 4 partners and the flags of the pages to walk: a
+    // flag at bit 8p+7 gives the page (p = bit >> 3) and the partner's shift
+    // (bit - 7).
     const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + G::kPartnerOff)[i];
-    if (MF_SYM_WALK_ROUNDS) {
-        uint32_t act = lds[G::kActOff + i];   // pages whose partner of cell i is already placed
+    uint32_t act = ew & 0x80808080u;
+    if constexpr (walk_mode<G>() == 2) {
         while (act != 0 && good != 0) {
-            const uint32_t p = static_cast<uint32_t>(__builtin_ctz(act));
+            const uint32_t b1 = static_cast<uint32_t>(__builtin_ctz(act));
+            act &= act - 1u;
+            const uint32_t a1 = (ew >> (b1 - 7u)) & 0x7fu, r1 = page_row<G>(b1 >> 3);
+            uint32_t j1 = lds[cyc_addr(cl, a1)];
+            Mask reach1 = static_cast<Mask>(1) << a1;
+            uint32_t a2 = a1, r2 = r1, j2 = a1;   // no second walk: closed from the start
+            Mask reach2 = ~static_cast<Mask>(0);
+            const bool two = act != 0;
+            if (two) {
+                const uint32_t b2 = static_cast<uint32_t>(__builtin_ctz(act));
+                act &= act - 1u;
+                a2 = (ew >> (b2 - 7u)) & 0x7fu;
+                r2 = page_row<G>(b2 >> 3);
+                j2 = lds[cyc_addr(cl, a2)];
+                reach2 = static_cast<Mask>(1) << a2;
+            }
+            uint32_t s1 = 0, s2 = 0;
+            while (j1 != a1 || j2 != a2) {
+                const bool m1 = j1 != a1, m2 = j2 != a2;
+                s1 += m1 ? 1u : 0u;
+                s2 += m2 ? 1u : 0u;
+                const uint32_t h1 = hop<G>(r1, i, j1);
+                const uint32_t h2 = hop<G>(r2, i, j2);
+                const uint32_t n1 = lds[cyc_addr(cl, h1)];
+                const uint32_t n2 = lds[cyc_addr(cl, h2)];
+                reach1 |= static_cast<Mask>(m1 ? 1u : 0u) << h1;
+                reach2 |= static_cast<Mask>(m2 ? 1u : 0u) << h2;
+                j1 = m1 ? n1 : j1;
+                j2 = m2 ? n2 : j2;
+            }
+            // Event counts as the one-page-at-a-time order: the second walk
+            // happens only if the first left good cells.
+            good &= reach1;
+            ++walks;
+            steps += s1;
+            if (two && good != 0) {
+                ++walks;
+                steps += s2;
+            }
+            good &= reach2;
+        }
+        return good;
+    }
+    if constexpr (walk_mode<G>() == 1) {
+        while (act != 0 && good != 0) {
+            const uint32_t b = static_cast<uint32_t>(__builtin_ctz(act));
             act &= act - 1u;
             ++walks;
-            good &= page_walk<G>(lds, cl, i, page_row<G>(p), (ew >> (8u * p)) & 0xffu, steps);
+            good &= page_walk<G>(lds, cl, i, page_row<G>(b >> 3), (ew >> (b - 7u)) & 0x7fu, steps);
         }
         return good;
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        const uint32_t anchor = (ew >> (8 * p)) & 0xffu;
+        const uint32_t anchor = (ew >> (8 * p)) & 0x7fu;
         if (anchor < i && good != 0) {
             ++walks;
             good &= page_walk<G>(lds, cl, i, page_row<G>(p), anchor, steps);
@@ -356,15 +413,15 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
     const auto *local = (const __attribute__((address_space(3))) uint8_t *)lds;
     if (tid == 0 && static_cast<uint32_t>(reinterpret_cast<size_t>(local)) != 0u) atomicOr(a.acc + ACC_ERROR, 4ull);
     if (tid < G::kCells) {
+        // Partners are at most n2 <= 81 < 128: bit 7 of each byte is free.
         const uint32_t w = a.edge_word[tid];
-        reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = w;
-        uint32_t act = 0;
+        uint32_t flagged = w;
 #pragma unroll
         for (uint32_t p = 0; p < 4; ++p) {
             lds[G::kPageOff + G::kCells * p + tid] = static_cast<uint8_t>(w >> (8u * p));
-            act |= (((w >> (8u * p)) & 0xffu) < tid ? 1u : 0u) << p;
+            flagged |= (((w >> (8u * p)) & 0xffu) < tid ? 0x80u : 0u) << (8u * p);
         }
-        lds[G::kActOff + tid] = static_cast<uint8_t>(act);
+        reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = flagged;
     }
     __syncthreads();
 }
