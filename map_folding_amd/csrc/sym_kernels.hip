@@ -261,11 +261,18 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
 // Backtracking pops at most this many levels per step; a lane with more to
 // pop skips the next step's walks and continues.  Bounding the pops keeps one
 // lane's long unwind from holding the whole wave (a lane pops 1.0 levels per
-// node on average, the deepest lane of a wave ~9).
+// node on average, the deepest lane of a wave ~9).  Each popped level costs
+// one or two dependent LDS round trips.  With paired walks a bound of 3 runs
+// the 8x8 depth-36 chunk 1% faster than 4 (5: 3% slower) and 7x7 1.2% faster;
+// 9x9, on one walk per round, is 1.5% faster with 4.  MF_SYM_MAX_POPS > 0
+// overrides the per-geometry bound.
 #ifndef MF_SYM_MAX_POPS
-#define MF_SYM_MAX_POPS 4
+#define MF_SYM_MAX_POPS 0
 #endif
-constexpr uint32_t kMaxPops = MF_SYM_MAX_POPS;
+template <typename G>
+constexpr uint32_t max_pops() {
+    return MF_SYM_MAX_POPS > 0 ? MF_SYM_MAX_POPS : (sizeof(typename G::Mask) > 8 ? 4u : 3u);
+}
 
 template <typename G, bool EV>
 __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t last, Work &w) {
@@ -298,7 +305,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             return false;
         }
     }
-    for (uint32_t k = 0; k < kMaxPops; ++k) {
+    for (uint32_t k = 0; k < max_pops<G>(); ++k) {
         if (w.i == w.root) {
             w.popping = false;
             return true;
