@@ -137,11 +137,12 @@ struct Totals {
 //      wave runs max(#pages) rounds; the page offset becomes a per-lane
 //      value (12% less time than mode 0 on the 8x8 depth-36 chunk);
 //   2  pairs: a round walks two of the lane's pages in one loop.  Both
-//      chains step every iteration, a closed one in place (its loads stay
-//      on valid cells, its results are discarded), so that the two chains'
-//      LDS loads are in flight together: a walk step is two dependent LDS
-//      loads, and at 5 waves per SIMD their latency, not VALU issue, bounds
-//      the loop.  5% less time than mode 1 on the 8x8 chunk.
+//      chains step every iteration, a closed one in place, so that the two
+//      chains' LDS loads are in flight together: a walk step is two
+//      dependent LDS loads, and at 5 waves per SIMD their latency, not VALU
+//      issue, bounds the loop.  A closed chain repeats its last step, which
+//      changes nothing, so it needs no mask: 9 VALU per chain step.  10%
+//      less time than mode 1 on the 8x8 chunk.
 // The default is pairs for u64 masks and rounds for u128 masks (9x9), where
 // two 128-bit bit-sets per iteration cost more than the overlap saves (5.6%
 // slower than rounds).  MF_SYM_WALK_MODE overrides it.
@@ -181,7 +182,7 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
     return reach;
 }
 
-template <typename G>
+template <typename G, bool EVC>
 __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
                                                       uint32_t &walks, uint32_t &steps) {
     using Mask = typename G::Mask;
@@ -198,8 +199,10 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             const uint32_t a1 = (ew >> (b1 - 7u)) & 0x7fu, r1 = page_row<G>(b1 >> 3);
             uint32_t j1 = lds[cyc_addr(cl, a1)];
             Mask reach1 = static_cast<Mask>(1) << a1;
-            uint32_t a2 = a1, r2 = r1, j2 = a1;   // no second walk: closed from the start
-            Mask reach2 = ~static_cast<Mask>(0);
+            // No second page: the second chain repeats the first (same cells,
+            // same reach, not counted), which leaves `good` unchanged.
+            uint32_t a2 = a1, r2 = r1, j2 = j1;
+            Mask reach2 = reach1;
             const bool two = act != 0;
             if (two) {
                 const uint32_t b2 = static_cast<uint32_t>(__builtin_ctz(act));
@@ -209,20 +212,29 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 j2 = lds[cyc_addr(cl, a2)];
                 reach2 = static_cast<Mask>(1) << a2;
             }
+            // A chain that closes keeps its last cell j: stepping it again
+            // hops to the same cell (already in reach) and back to the
+            // anchor, so a closed chain needs no mask.  A walk of no steps
+            // (the anchor alone in the cycle, so its partner is not placed)
+            // hops from the anchor to itself: the same.
             uint32_t s1 = 0, s2 = 0;
-            while (j1 != a1 || j2 != a2) {
-                const bool m1 = j1 != a1, m2 = j2 != a2;
-                s1 += m1 ? 1u : 0u;
-                s2 += m2 ? 1u : 0u;
+            bool o1 = j1 != a1, o2 = j2 != a2;
+            do {
+                if (EVC) {
+                    s1 += o1 ? 1u : 0u;
+                    s2 += o2 ? 1u : 0u;
+                }
                 const uint32_t h1 = hop<G>(r1, i, j1);
                 const uint32_t h2 = hop<G>(r2, i, j2);
                 const uint32_t n1 = lds[cyc_addr(cl, h1)];
                 const uint32_t n2 = lds[cyc_addr(cl, h2)];
-                reach1 |= static_cast<Mask>(m1 ? 1u : 0u) << h1;
-                reach2 |= static_cast<Mask>(m2 ? 1u : 0u) << h2;
-                j1 = m1 ? n1 : j1;
-                j2 = m2 ? n2 : j2;
-            }
+                reach1 |= static_cast<Mask>(1) << h1;
+                reach2 |= static_cast<Mask>(1) << h2;
+                o1 = n1 != a1;
+                o2 = n2 != a2;
+                j1 = o1 ? n1 : j1;
+                j2 = o2 ? n2 : j2;
+            } while (o1 || o2);
             // Event counts as the one-page-at-a-time order: the second walk
             // happens only if the first left good cells.
             good &= reach1;
@@ -279,7 +291,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
     using Stack = typename G::Stack;
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
-        typename G::Mask g = good_mask<G>(lds, ln.cl, w.i, nw, ns);
+        typename G::Mask g = good_mask<G, EV>(lds, ln.cl, w.i, nw, ns);
         uint32_t cnt;   // children of this node
         if (w.floor == 0) {
             ++w.inodes;
