@@ -141,7 +141,7 @@ struct Totals {
 //      chains' LDS loads are in flight together: a walk step is two
 //      dependent LDS loads, and at 5 waves per SIMD their latency, not VALU
 //      issue, bounds the loop.  A closed chain repeats its last step, which
-//      changes nothing, so it needs no mask: 9 VALU per chain step.  10%
+//      changes nothing, so it needs no mask: 9 VALU per chain step.  13%
 //      less time than mode 1 on the 8x8 chunk.
 // The default is pairs for u64 masks and rounds for u128 masks (9x9), where
 // two 128-bit bit-sets per iteration cost more than the overlap saves (5.6%
@@ -149,6 +149,12 @@ struct Totals {
 // overrides it.
 #ifndef MF_SYM_WALK_MODE
 #define MF_SYM_WALK_MODE -1
+#endif
+// Paired steps per loop trip: a closed chain's extra steps change nothing,
+// so the exit test runs every second step (3% less time on the 8x8 and 7x7
+// chunks than every step; three per trip is slower again).
+#ifndef MF_SYM_PAIR_UNROLL
+#define MF_SYM_PAIR_UNROLL 2
 #endif
 template <typename G>
 constexpr int walk_mode() {
@@ -221,20 +227,23 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             uint32_t s1 = 0, s2 = 0;
             bool o1 = j1 != a1, o2 = j2 != a2;
             do {
-                if (EVC) {
-                    s1 += o1 ? 1u : 0u;
-                    s2 += o2 ? 1u : 0u;
+#pragma unroll
+                for (int u = 0; u < MF_SYM_PAIR_UNROLL; ++u) {
+                    if (EVC) {
+                        s1 += o1 ? 1u : 0u;
+                        s2 += o2 ? 1u : 0u;
+                    }
+                    const uint32_t h1 = hop<G>(r1, i, j1);
+                    const uint32_t h2 = hop<G>(r2, i, j2);
+                    const uint32_t n1 = lds[cyc_addr(cl, h1)];
+                    const uint32_t n2 = lds[cyc_addr(cl, h2)];
+                    reach1 |= static_cast<Mask>(1) << h1;
+                    reach2 |= static_cast<Mask>(1) << h2;
+                    o1 = n1 != a1;
+                    o2 = n2 != a2;
+                    j1 = o1 ? n1 : j1;
+                    j2 = o2 ? n2 : j2;
                 }
-                const uint32_t h1 = hop<G>(r1, i, j1);
-                const uint32_t h2 = hop<G>(r2, i, j2);
-                const uint32_t n1 = lds[cyc_addr(cl, h1)];
-                const uint32_t n2 = lds[cyc_addr(cl, h2)];
-                reach1 |= static_cast<Mask>(1) << h1;
-                reach2 |= static_cast<Mask>(1) << h2;
-                o1 = n1 != a1;
-                o2 = n2 != a2;
-                j1 = o1 ? n1 : j1;
-                j2 = o2 ? n2 : j2;
             } while (o1 || o2);
             // Event counts as the one-page-at-a-time order: the second walk
             // happens only if the first left good cells.
