@@ -20,13 +20,25 @@ __device__ __forceinline__ uint64_t below_mask(uint32_t i) { return (i >= 64u) ?
 // reading different entries can meet in a bank).  The dword-interleaved
 // alternative (dword w of lane L at word w*64 + L: conflict-free, four VALU ops
 // per address) measured 11% slower on the 8x8 depth-36 chunk.
-__device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) { return lane_base + (c << 6); }
+// Entry c of a lane's row sits kStride bytes after entry c-1.  A stride that
+// is not a power of two is one 24-bit multiply-add (c < 2^24); written out,
+// because the compiler otherwise forms a 64-bit multiply-add for it.
+template <uint32_t kStride = 64>
+__device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) {
+    if constexpr ((kStride & (kStride - 1)) == 0) {
+        return lane_base + c * kStride;
+    } else {
+        uint32_t r;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(c), "s"(kStride), "v"(lane_base));
+        return r;
+    }
+}
 __device__ __forceinline__ uint32_t row_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
 
 // Copy a work item's first kBytes state bytes (a multiple of 16) into the
 // lane's row, and back.  The row holds kRow entries (kRow <= kBytes); state
 // bytes past it are zero in every item (cells past n*n) and are not stored.
-template <uint32_t kBytes = 64, uint32_t kRow = kBytes>
+template <uint32_t kBytes = 64, uint32_t kRow = kBytes, uint32_t kStride = 64>
 __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_t *src) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
 #pragma unroll
@@ -35,11 +47,11 @@ __device__ __forceinline__ void row_load(uint8_t *lds, uint32_t lb, const uint8_
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k)
-            if (16u * q + k < kRow) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+            if (16u * q + k < kRow) lds[lb + (16u * q + k) * kStride] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
     }
 }
 
-template <uint32_t kBytes = 64, uint32_t kRow = kBytes>
+template <uint32_t kBytes = 64, uint32_t kRow = kBytes, uint32_t kStride = 64>
 __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8_t *dst) {
     uint4 *d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
@@ -48,7 +60,7 @@ __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k)
             if (16u * q + k < kRow)
-                w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+                w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * kStride]) << (8 * (k & 3));
         d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }

@@ -3,8 +3,10 @@
 // Restates recurse_count / get_good / iter_good (sym_fold.cpp:179-233) as a
 // per-lane iterative DFS:
 //   * one lane owns one subtree (a WorkItem);
-//   * its cycle (cell -> successor, bytes) lives in LDS, lane-interleaved so
-//     that lanes never share a bank (dword w of lane L at word w*64 + L);
+//   * its cycle (cell -> successor, bytes) lives in LDS, byte-interleaved
+//     across the wave's lanes (entry c of lane L at byte c*S + L, S = 68 for
+//     the u64 instances so that quad lanes reading different cells fall in
+//     different banks, 64 for 9x9);
 //   * the partner of every cell on each of the 4 pages sits in an LDS table
 //     (byte p of word c);
 //   * node = one get_good call (sym_fold9.cpp:342-344); a node at the last
@@ -52,17 +54,27 @@ namespace {
 // partner is below the cell, i.e. get_good walks page p for it), the
 // partners page-major (byte kPageOff + kCells*p + c), then per wave a
 // kCells-entry byte row per lane and the stack.
-template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_>
+// Row stride of the u64 geometries: 68 bytes (17 dwords, odd) instead of 64
+// puts the 4 lanes of a quad in different banks when they read different
+// cells; 2.3% less time on the 8x8 chunk and 3.5% on 7x7, for 4 more bytes
+// per entry (5 blocks per CU at 8x8 depth 36 and 6 at 7x7 depth 24, as with
+// 64).  9x9 keeps 64: its 81-entry rows would lose a block per CU.
+#ifndef MF_SYM_ROW_STRIDE
+#define MF_SYM_ROW_STRIDE 68
+#endif
+template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_,
+          uint32_t kRowStride_ = MF_SYM_ROW_STRIDE>
 struct Geometry {
     static constexpr uint32_t kCells = kCells_;
     static constexpr uint32_t kRowCells = kRowCells_;   // entries of a lane's LDS row (cells of the largest map)
+    static constexpr uint32_t kRowStride = kRowStride_;   // bytes from entry c to entry c+1 of a lane
     using Mask = Mask_;
     using Item = Item_;
     using Stack = Stack_;
     static constexpr uint32_t kPartnerOff = 0;
     static constexpr uint32_t kPageOff = 4 * kCells;
     static constexpr uint32_t kTableBytes = 8 * kCells;
-    static constexpr uint32_t kCycleBytes = kRowCells * 64;    // one row of kRowCells bytes per lane
+    static constexpr uint32_t kCycleBytes = kRowCells * kRowStride;   // every lane's row
     static constexpr uint32_t kStateBytes = kCells;
     // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
     // j < i, so it stays under the mask width (64 up to 8x8, 128 for 9x9).
@@ -82,7 +94,7 @@ using G64 = Geometry<64, uint64_t, WorkItem, CompactStack>;
 using G64Wide = Geometry<64, uint64_t, WorkItem, WideStack>;
 // 9x9: 81 cells in LDS rows (5,184 B per wave instead of 6,144), so four
 // 4-wave blocks fit a CU at depth 44 instead of three.
-using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128, 81>;
+using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128, 81, 64>;
 // Up to 7x7: 49-entry rows (3,136 B per wave): six 4-wave blocks per CU at
 // 7x7 depth 24 instead of five.
 using G49 = Geometry<64, uint64_t, WorkItem, CompactStack, 49>;
@@ -95,17 +107,44 @@ inline Geo geometry(int core, uint32_t cells) {
     return cells <= 49 ? Geo::k49 : Geo::k64;
 }
 
-__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) { return row_at(lane_base, cell); }
+template <typename G>
+__device__ __forceinline__ uint32_t cyc_addr(uint32_t lane_base, uint32_t cell) {
+    return row_at<G::kRowStride>(lane_base, cell);
+}
+// Cycle entry of `cell` in the lane's row.  With a stride that is not a power
+// of two the entry is read by absolute 32-bit LDS address (the kernel's only
+// LDS is the dynamic block at offset 0, checked when the tables are staged),
+// which keeps the address a single 24-bit multiply-add; a power-of-two stride
+// indexes the block, which lets the row offset fold into the ds instruction.
+template <typename G>
+__device__ __forceinline__ uint32_t cyc_ld(const uint8_t *lds, uint32_t lane_base, uint32_t cell) {
+    if constexpr ((G::kRowStride & (G::kRowStride - 1)) == 0) {
+        return lds[cyc_addr<G>(lane_base, cell)];
+    } else {
+        return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(cyc_addr<G>(lane_base, cell));
+    }
+}
+template <typename G>
+__device__ __forceinline__ void cyc_st(uint8_t *lds, uint32_t lane_base, uint32_t cell, uint32_t v) {
+    if constexpr ((G::kRowStride & (G::kRowStride - 1)) == 0) {
+        lds[cyc_addr<G>(lane_base, cell)] = static_cast<uint8_t>(v);
+    } else {
+        *reinterpret_cast<__attribute__((address_space(3))) uint8_t *>(cyc_addr<G>(lane_base, cell)) =
+            static_cast<uint8_t>(v);
+    }
+}
 
 // Insert cell i after cell j (iter_good, sym_fold.cpp:210-213).
+template <typename G>
 __device__ __forceinline__ void place(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
-    const uint8_t nx = lds[cyc_addr(ln.cl, j)];
-    lds[cyc_addr(ln.cl, i)] = nx;
-    lds[cyc_addr(ln.cl, j)] = static_cast<uint8_t>(i);
+    const uint32_t nx = cyc_ld<G>(lds, ln.cl, j);
+    cyc_st<G>(lds, ln.cl, i, nx);
+    cyc_st<G>(lds, ln.cl, j, i);
 }
 // Remove cell i again (sym_fold.cpp:215): its successor goes back to j.
+template <typename G>
 __device__ __forceinline__ void unplace(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
-    lds[cyc_addr(ln.cl, j)] = lds[cyc_addr(ln.cl, i)];
+    cyc_st<G>(lds, ln.cl, j, cyc_ld<G>(lds, ln.cl, i));
 }
 
 struct Work {
@@ -179,12 +218,12 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
                                                       uint32_t prow, uint32_t anchor, uint32_t &steps) {
     using Mask = typename G::Mask;
     Mask reach = static_cast<Mask>(1) << anchor;
-    uint32_t j = lds[cyc_addr(cl, anchor)];
+    uint32_t j = cyc_ld<G>(lds, cl, anchor);
     while (j != anchor) {
         ++steps;
         j = hop<G>(prow, i, j);
         reach |= static_cast<Mask>(1) << j;
-        j = lds[cyc_addr(cl, j)];
+        j = cyc_ld<G>(lds, cl, j);
     }
     return reach;
 }
@@ -204,7 +243,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             const uint32_t b1 = static_cast<uint32_t>(__builtin_ctz(act));
             act &= act - 1u;
             const uint32_t a1 = (ew >> (b1 - 7u)) & 0x7fu, r1 = page_row<G>(b1 >> 3);
-            uint32_t j1 = lds[cyc_addr(cl, a1)];
+            uint32_t j1 = cyc_ld<G>(lds, cl, a1);
             Mask reach1 = static_cast<Mask>(1) << a1;
             // No second page: the second chain repeats the first (same cells,
             // same reach, not counted), which leaves `good` unchanged.
@@ -216,7 +255,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 act &= act - 1u;
                 a2 = (ew >> (b2 - 7u)) & 0x7fu;
                 r2 = page_row<G>(b2 >> 3);
-                j2 = lds[cyc_addr(cl, a2)];
+                j2 = cyc_ld<G>(lds, cl, a2);
                 reach2 = static_cast<Mask>(1) << a2;
             }
             // A chain that closes keeps its last cell j: stepping it again
@@ -235,8 +274,8 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                     }
                     const uint32_t h1 = hop<G>(r1, i, j1);
                     const uint32_t h2 = hop<G>(r2, i, j2);
-                    const uint32_t n1 = lds[cyc_addr(cl, h1)];
-                    const uint32_t n2 = lds[cyc_addr(cl, h2)];
+                    const uint32_t n1 = cyc_ld<G>(lds, cl, h1);
+                    const uint32_t n2 = cyc_ld<G>(lds, cl, h2);
                     reach1 |= static_cast<Mask>(1) << h1;
                     reach2 |= static_cast<Mask>(1) << h2;
                     o1 = n1 != a1;
@@ -322,7 +361,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         if (g != 0) {
             const typename Stack::Entry e = Stack::encode(g, cnt);
             Stack::put(lds, ln, w.i, e);
-            place(lds, ln, Stack::child(e), w.i);
+            place<G>(lds, ln, Stack::child(e), w.i);
             ++w.i;
             return false;
         }
@@ -335,12 +374,12 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         --w.i;
         const typename Stack::Entry e = Stack::get(lds, ln, w.i);
         const uint32_t j = Stack::child(e);
-        unplace(lds, ln, j, w.i);
+        unplace<G>(lds, ln, j, w.i);
         const uint32_t more = Stack::more(e);
         if (more == 1u) {
             const typename Stack::Entry e2 = Stack::advance(e);
             Stack::put(lds, ln, w.i, e2);
-            place(lds, ln, Stack::child(e2), w.i);
+            place<G>(lds, ln, Stack::child(e2), w.i);
             ++w.i;
             w.popping = false;
             return false;
@@ -363,7 +402,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
 template <typename G>
 __device__ __forceinline__ void load_item_common(uint8_t *lds, const Lane &ln,
                                                  const typename G::Item *__restrict__ it, Work &w) {
-    row_load<G::kStateBytes, G::kRowCells>(lds, ln.cl, it->state);
+    row_load<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, it->state);
     w.origin = it->origin;
     w.mult = it->mult;
     w.root = it->level;
@@ -392,7 +431,7 @@ __device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const ty
         // records only part of it and recomputes the rest is exact.
         const typename Stack::Entry e = Stack::encode(mask);
         Stack::put(lds, ln, level, e);
-        place(lds, ln, Stack::child(e), level);
+        place<G>(lds, ln, Stack::child(e), level);
         w.i = level + 1;
     }
 }
@@ -477,7 +516,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         if (emit) {
             if (slot < a.spill_cap) {
                 Item *dst = static_cast<Item *>(a.spill) + slot;
-                row_store<G::kStateBytes, G::kRowCells>(lds, ln.cl, dst->state);
+                row_store<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, dst->state);
                 G::set_item_mask(dst, mask);
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.mult);
@@ -496,7 +535,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
                 --t;
                 const typename Stack::Entry e = Stack::get(lds, ln, t);
                 const uint32_t j = Stack::child(e);
-                unplace(lds, ln, j, t);
+                unplace<G>(lds, ln, j, t);
                 bool unknown = false;
                 mask = Stack::rest(e, unknown);
                 floor = 0;
