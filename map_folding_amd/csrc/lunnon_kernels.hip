@@ -57,14 +57,20 @@ struct LTotals {
     unsigned long long nodes = 0, walks = 0, steps = 0, cands = 0, comps = 0, items = 0;
 };
 
+// Byte stride between entries of a lane's link row.
+#ifndef MF_LUNNON_ROW_STRIDE
+#define MF_LUNNON_ROW_STRIDE 64
+#endif
+constexpr uint32_t kLStride = MF_LUNNON_ROW_STRIDE;
+
 // Link leaf l directly below leaf m (mf.c:137-140 without a[]).
 __device__ __forceinline__ void link(uint8_t *lds, const Lane &ln, uint32_t m, uint32_t l) {
-    lds[row_at(ln.cl, l)] = lds[row_at(ln.cl, m)];
-    lds[row_at(ln.cl, m)] = static_cast<uint8_t>(l);
+    row_st<kLStride>(lds, row_at<kLStride>(ln.cl, l), row_ld<kLStride>(lds, row_at<kLStride>(ln.cl, m)));
+    row_st<kLStride>(lds, row_at<kLStride>(ln.cl, m), l);
 }
 // Unlink leaf l that sits below leaf m (mf.c:131-132).
 __device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m, uint32_t l) {
-    lds[row_at(ln.cl, m)] = lds[row_at(ln.cl, l)];
+    row_st<kLStride>(lds, row_at<kLStride>(ln.cl, m), row_ld<kLStride>(lds, row_at<kLStride>(ln.cl, l)));
 }
 
 // Legal positions for leaf l: mask of m < l reached by the walk of every
@@ -93,7 +99,7 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         do {
             ++steps;
             reach |= 1ull << m;
-            m = lds[row + lds[row_at(ln.cl, m)]];
+            m = lds[row + row_ld<kLStride>(lds, row_at<kLStride>(ln.cl, m))];
         } while (m != l);
         meet &= reach;
         reach_any |= reach;
@@ -172,7 +178,7 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
 template <typename Stack>
 __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkItem *__restrict__ it,
                                            const LunnonMap *maps, uint32_t rows, LWork &w) {
-    row_load_n(lds, ln.cl, it->state, rows);
+    row_load_n<kLStride>(lds, ln.cl, it->state, rows);
     const LunnonMap mp = maps[it->map];
     w.n = mp.n;
     w.ndim = mp.ndim;
@@ -238,7 +244,7 @@ __device__ __forceinline__ uint32_t tables_span(uint32_t bytes) { return (bytes 
 template <typename Stack>
 __device__ __forceinline__ Lane lmake_lane(uint32_t tab_span, uint32_t levels, uint32_t rows) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t row_bytes = rows * 64u;
+    const uint32_t row_bytes = (rows * kLStride + 15u) & ~15u;   // the stack after it stays 16-byte aligned
     const uint32_t wbase = tab_span + kMapBytes + wave * (row_bytes + levels * Stack::kSlotBytes);
     Lane ln;
     ln.cl = row_lane_base(wbase, lane);
@@ -278,7 +284,7 @@ __device__ __forceinline__ void lspill_all(const LunnonArgs &a, uint8_t *lds, co
         if (emit) {
             if (slot < a.spill_cap) {
                 WorkItem *dst = a.spill + slot;
-                row_store_n(lds, ln.cl, dst->state, a.row_entries);
+                row_store_n<kLStride>(lds, ln.cl, dst->state, a.row_entries);
                 dst->mask = mask;
                 dst->origin = w.origin;
                 dst->mult = static_cast<uint16_t>(w.n | (floor << 8));
@@ -426,7 +432,7 @@ const void *lpick(bool persistent, bool events) {
 
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows) {
     const uint32_t slot = core == 1 ? WideStack::kSlotBytes : CompactStack::kSlotBytes;
-    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (rows * 64u + levels * slot);
+    return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (((rows * kLStride + 15u) & ~15u) + levels * slot);
 }
 
 // core: 1 = wide u64 stack, otherwise the compact 16-bit stack.

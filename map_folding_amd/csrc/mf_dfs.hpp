@@ -35,6 +35,27 @@ __device__ __forceinline__ uint32_t row_at(uint32_t lane_base, uint32_t c) {
 }
 __device__ __forceinline__ uint32_t row_lane_base(uint32_t wbase, uint32_t lane) { return wbase + lane; }
 
+// Byte of a lane row at `addr`: a power-of-two stride indexes the block (the
+// row offset folds into the ds instruction); otherwise the absolute 32-bit
+// LDS address is used (the kernels' only LDS is the dynamic block at offset 0),
+// which keeps the address arithmetic 32-bit.
+template <uint32_t kStride>
+__device__ __forceinline__ uint32_t row_ld(const uint8_t *lds, uint32_t addr) {
+    if constexpr ((kStride & (kStride - 1)) == 0) {
+        return lds[addr];
+    } else {
+        return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(addr);
+    }
+}
+template <uint32_t kStride>
+__device__ __forceinline__ void row_st(uint8_t *lds, uint32_t addr, uint32_t v) {
+    if constexpr ((kStride & (kStride - 1)) == 0) {
+        lds[addr] = static_cast<uint8_t>(v);
+    } else {
+        *reinterpret_cast<__attribute__((address_space(3))) uint8_t *>(addr) = static_cast<uint8_t>(v);
+    }
+}
+
 // Copy a work item's first kBytes state bytes (a multiple of 16) into the
 // lane's row, and back.  The row holds kRow entries (kRow <= kBytes); state
 // bytes past it are zero in every item (cells past n*n) and are not stored.
@@ -67,6 +88,7 @@ __device__ __forceinline__ void row_store(const uint8_t *lds, uint32_t lb, uint8
 
 // The same for a row of `rows` entries known only at run time (rows <= 64):
 // entries past it are zero in every item and are not stored.
+template <uint32_t kStride = 64>
 __device__ __forceinline__ void row_load_n(uint8_t *lds, uint32_t lb, const uint8_t *src, uint32_t rows) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
 #pragma unroll
@@ -75,10 +97,11 @@ __device__ __forceinline__ void row_load_n(uint8_t *lds, uint32_t lb, const uint
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k)
-            if (16u * q + k < rows) lds[lb + (16u * q + k) * 64u] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
+            if (16u * q + k < rows) lds[lb + (16u * q + k) * kStride] = static_cast<uint8_t>(w[k >> 2] >> (8 * (k & 3)));
     }
 }
 
+template <uint32_t kStride = 64>
 __device__ __forceinline__ void row_store_n(const uint8_t *lds, uint32_t lb, uint8_t *dst, uint32_t rows) {
     uint4 *d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
@@ -86,7 +109,7 @@ __device__ __forceinline__ void row_store_n(const uint8_t *lds, uint32_t lb, uin
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k)
-            if (16u * q + k < rows) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * 64u]) << (8 * (k & 3));
+            if (16u * q + k < rows) w[k >> 2] |= static_cast<uint32_t>(lds[lb + (16u * q + k) * kStride]) << (8 * (k & 3));
         d4[q] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }

@@ -74,7 +74,8 @@ struct Geometry {
     static constexpr uint32_t kPartnerOff = 0;
     static constexpr uint32_t kPageOff = 4 * kCells;
     static constexpr uint32_t kTableBytes = 8 * kCells;
-    static constexpr uint32_t kCycleBytes = kRowCells * kRowStride;   // every lane's row
+    // Every lane's row, rounded so that the stack after it stays 16-byte aligned.
+    static constexpr uint32_t kCycleBytes = (kRowCells * kRowStride + 15u) & ~15u;
     static constexpr uint32_t kStateBytes = kCells;
     // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
     // j < i, so it stays under the mask width (64 up to 8x8, 128 for 9x9).
