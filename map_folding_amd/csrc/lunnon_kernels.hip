@@ -185,6 +185,7 @@ __device__ __forceinline__ void lload_item(uint8_t *lds, Lane &ln, const WorkIte
     w.tab = mp.table_off;
     w.tri = ((mp.n + 1u) * (mp.n + 2u)) >> 1;
     ln.base = mp.base;
+    ln.s0 = ln.sl - mp.base * Stack::kSlotBytes;
     w.map = it->map;
     w.origin = it->origin;
     const uint32_t level = it->level;
@@ -250,6 +251,7 @@ __device__ __forceinline__ Lane lmake_lane(uint32_t tab_span, uint32_t levels, u
     ln.cl = row_lane_base(wbase, lane);
     ln.sl = wbase + row_bytes + lane * Stack::kLaneBytes;
     ln.base = 0;
+    ln.s0 = ln.sl;
     return ln;
 }
 

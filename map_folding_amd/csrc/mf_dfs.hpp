@@ -130,6 +130,7 @@ struct Lane {
     uint32_t cl;      // LDS byte address of this lane's row entry 0
     uint32_t sl;      // LDS byte address of this lane's stack slot 0
     uint32_t base;    // level held by stack slot 0
+    uint32_t s0;      // sl - base * slot bytes (mod 2^32): level l's slot is s0 + l * slot bytes
 };
 
 // ---------------------------------------------------------- stack encodings
@@ -144,10 +145,10 @@ struct WideStack {
     using Entry = uint64_t;
     using MaskT = uint64_t;
     static __device__ __forceinline__ uint64_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-        return *reinterpret_cast<const uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+        return *reinterpret_cast<const uint64_t *>(lds + (ln.s0 + level * kSlotBytes));
     }
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t e) {
-        *reinterpret_cast<uint64_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = e;
+        *reinterpret_cast<uint64_t *>(lds + (ln.s0 + level * kSlotBytes)) = e;
     }
     static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
     static __device__ __forceinline__ uint64_t encode(uint64_t g, uint32_t) { return g; }
@@ -177,10 +178,10 @@ struct CompactStackT {
     using Entry = uint32_t;
     using MaskT = Mask;
     static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-        return *reinterpret_cast<const uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes);
+        return *reinterpret_cast<const uint16_t *>(lds + (ln.s0 + level * kSlotBytes));
     }
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
-        *reinterpret_cast<uint16_t *>(lds + ln.sl + (level - ln.base) * kSlotBytes) = static_cast<uint16_t>(e);
+        *reinterpret_cast<uint16_t *>(lds + (ln.s0 + level * kSlotBytes)) = static_cast<uint16_t>(e);
     }
     static __device__ __forceinline__ uint32_t encode(Mask g) {
         const uint32_t j = mask_ctz(g);

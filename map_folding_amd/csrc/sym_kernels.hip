@@ -472,6 +472,7 @@ __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) 
     ln.cl = row_lane_base(wbase, lane);
     ln.sl = wbase + G::kCycleBytes + lane * G::Stack::kLaneBytes;
     ln.base = base_level;
+    ln.s0 = ln.sl - base_level * G::Stack::kSlotBytes;
     return ln;
 }
 
