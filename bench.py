@@ -229,7 +229,7 @@ def run_lunnon(args, mf, rl):
             from map_folding_amd import autotune
             t1 = time.time()
             cfg = autotune.tune_lunnon_batch(shapes, targets=(256, 1024, 4096, 16384), budgets=(32, 64, 128),
-                                             waves=(16, 32), log=log)
+                                             waves=(16, 32), log=log, extra_depths=[depths])
             depths = cfg["depths"]
             batched = mf.SolveOptions(schedule=mf.MF_SCHED_RESPLIT, count_events=False, budget=cfg["budget"],
                                       waves_per_cu=cfg["waves_per_cu"], core=args.core)

@@ -51,9 +51,10 @@ def depths_for_target(shapes, target: int, max_states: int = 2_000_000):
 
 
 def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 256), waves=(16, 32),
-                      repeats=2, log=None):
-    """Pick per-map frontier depths (by a target states-per-map), then the
-    re-split budget and resident waves, for one batched launch of `shapes`.
+                      repeats=2, log=None, extra_depths=()):
+    """Pick per-map frontier depths (by a target states-per-map, or one of the
+    explicit per-map depth lists in `extra_depths`), then the re-split budget
+    and resident waves, for one batched launch of `shapes`.
 
     Returns {"depths", "budget", "waves_per_cu", "kernel_ms", "trials"}."""
     say = log or (lambda m: None)
@@ -82,8 +83,10 @@ def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 25
         return ms
 
     best = None
-    for t in targets:
-        d = depths_for_target(shapes, t)
+    candidates = [depths_for_target(shapes, t) for t in targets] + [list(d) for d in extra_depths]
+    for d in candidates:
+        if len(d) != len(shapes):
+            raise ValueError("a depth list needs one depth per map")
         ms = solve_with(d, budgets[len(budgets) // 2], waves[-1])
         if best is None or ms < best[0]:
             best = (ms, d)

@@ -18,8 +18,15 @@ def test_depths_for_target(mf):
 def test_tune_lunnon_batch_keeps_totals(gpu, golden):
     from map_folding_amd import autotune
     shapes = [(3, 4), (2, 6), (4, 4), (3, 5), (2, 8)]
-    cfg = autotune.tune_lunnon_batch(shapes, targets=(16, 128), budgets=(16, 64), waves=(16, 32), repeats=1)
+    extra = [[6, 5, 8, 7, 9]]
+    cfg = autotune.tune_lunnon_batch(shapes, targets=(16, 128), budgets=(16, 64), waves=(16, 32), repeats=1,
+                                     extra_depths=extra)
     assert cfg["kernel_ms"] == min(t["kernel_ms"] for t in cfg["trials"])
+    # Every depth candidate (two targets and the explicit list) is timed once
+    # before the budget / waves sweep on the best of them.
+    assert [t["depths"] for t in cfg["trials"][:3]][2] == extra[0]
+    with pytest.raises(ValueError):
+        autotune.tune_lunnon_batch(shapes, targets=(), budgets=(16,), waves=(32,), repeats=1, extra_depths=[[6]])
     b = gpu.LunnonBatchFrontier(shapes, cfg["depths"])
     totals, _ = b.solve_maps(gpu.SolveOptions(budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"]))
     assert totals == [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
