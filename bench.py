@@ -179,12 +179,16 @@ def rect_depth(p, q):
 
 
 def roof_peak(rl, cus):
-    """The int32 VALU roof in T ops/s, measured on this device by valu_peak
-    (after the timed steps), with the spec-sheet figure beside it."""
-    ops, source = rl.measured_int32_peak(ROOT)
+    """The int32 VALU roof in T ops/s: the larger of valu_peak's reading on
+    this device (taken after the timed steps) and the best recorded reading,
+    so that a box that reads low does not inflate the fraction.  The live
+    reading and the spec-sheet figure are reported beside it."""
+    live, source = rl.measured_int32_peak(ROOT)
+    ops = max(live, rl.INT32_MEASURED_OPS)
     spec = rl.int32_peak_ops(cus or 256) / 1e12
-    log(f"int32 VALU peak: {ops:.4e} ops/s ({source}); CUs x 64 x 2.4 GHz = {spec*1e12:.4e}")
-    return ops / 1e12, {"peak_source": source, "peak_cus_x64_x2p4ghz": spec,
+    log(f"int32 VALU peak: {ops:.4e} ops/s (this box: {live:.4e}, {source}); CUs x 64 x 2.4 GHz = {spec*1e12:.4e}")
+    return ops / 1e12, {"peak_source": f"max(this box's reading, best recorded {rl.INT32_MEASURED_OPS:.3e}); {source}",
+                        "peak_live": live / 1e12, "peak_cus_x64_x2p4ghz": spec,
                         "peak_published_fp32_lane_fma": rl.PUBLISHED_FP32_LANE_OPS / 1e12}
 
 

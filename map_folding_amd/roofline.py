@@ -34,7 +34,7 @@ W_LUN_GAPCOMP, W_LUN_STEP, W_LUN_CAND, W_LUN_INSERT, W_LUN_COMPLETION = 10, 8, 6
 MI355X_CLOCK_HZ = 2.4e9          # peak engine clock
 LANES_PER_CU_PER_CLK = 64        # 4 SIMD x 16 lanes, one int32 op each per clock
 HBM_PEAK_GBS = 8000.0
-INT32_MEASURED_OPS = 6.19e13     # valu_peak's best reading on one MI355X (256 CUs)
+INT32_MEASURED_OPS = 6.19e13     # valu_peak's best reading on one MI355X (256 CUs); readings span 5.5-6.3e13
 PUBLISHED_FP32_LANE_OPS = 157.3e12 / 2
 
 
