@@ -367,9 +367,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on exactly one GPU: this workload has no multi-GPU path in this build "
-              "(chunks shard as independent case ranges with one 8-byte sum; see the design notes).",
-              file=sys.stderr)
+        print("bench.py runs on exactly one GPU: this workload has no multi-GPU path. A chunk would "
+              "shard as independent case ranges, one per GPU, whose exact counts add (one 16-byte "
+              "u128 sum per rank).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
