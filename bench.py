@@ -12,8 +12,8 @@ Prints ONE JSON line.  `value` is DFS nodes/s (node = one get_good call) of
 the batched persistent re-split schedule fed by the learned cost model: the
 model is trained on the GPU (pairs logged by the GPU DFS on case windows
 disjoint from the timed chunk), predicts every case, and cases predicted large
-are split into subtrees that are queued largest-first (outside the timed
-region, like the case generation; both times are reported).  The same
+are split into subtrees that are queued largest size class first, then by
+level (outside the timed region, like the case generation; both times are reported).  The same
 schedule without the model, and the plain static-frontier kernel (one lane
 per case, one launch) on the chunk's first --plain-cases cases (it is bounded
 by its longest case: minutes on the whole chunk), are timed beside it, the
@@ -424,7 +424,7 @@ def main():
 
     # Learned part: train the cost model on the GPU (pairs logged by the GPU DFS
     # on windows disjoint from this chunk), predict every case, split the ones
-    # predicted large into subtrees, order the pieces largest-first.
+    # predicted large into subtrees, lay the pieces out largest size class first.
     learned = None
     timed, timed_opts, gpu_nodes = front, batched, nodes
     n_inputs = cyc.shape[0]

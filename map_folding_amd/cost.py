@@ -20,7 +20,7 @@ from . import _abi
 from ._abi import MF_COST_FEATURES, MF_COST_PARAMS, MFCostTrainOptions, check, lib
 
 __all__ = [
-    "CostModel", "features_sym", "features_lunnon", "node_target", "learned_order", "lookahead_order",
+    "CostModel", "features_sym", "features_lunnon", "node_target", "learned_order", "class_order", "lookahead_order",
     "estimate_work", "sort_by_score", "wave_waste", "list_makespan", "spearman",
 ]
 
@@ -143,6 +143,22 @@ def learned_order(pred) -> np.ndarray:
     return np.argsort(-np.asarray(pred, dtype=np.float64), kind="stable").astype(np.uint32)
 
 
+def class_order(pred, levels, width: float = 2.0) -> np.ndarray:
+    """Largest size class first, where a class spans `width` units of predicted
+    log2 nodes; inside a class, by level, then in the given order.
+
+    The given order of split pieces is the splitter's: level by level, siblings
+    next to each other. Keeping siblings together inside a class puts subtrees
+    of the same depth and a similar shape in one wave, so its lanes' walks and
+    backtracking depths differ less than under a strict largest-first sort
+    (8x8 depth-36 chunk of 50,000 cases, 3.77M pieces: 962-964 ms at width 2
+    or 3, 968 ms at width 8, 987 ms strictly largest first, 982 ms in the
+    splitter's order alone)."""
+    p = np.asarray(pred, dtype=np.float64)
+    lv = np.asarray(levels, dtype=np.int64)
+    return np.lexsort((np.arange(len(p)), lv, -np.floor(p / width))).astype(np.uint32)
+
+
 def estimate_work(n: int, level: int, cycles: np.ndarray, depth: int) -> np.ndarray:
     """The reference's lookahead proxy: descendants at level min(n^2, level+depth)
     of each case (estimate_work, sym_fold.cpp:235-250); rows of case_bytes(n)."""
@@ -223,8 +239,8 @@ class Pieces:
         return len(self.levels)
 
     def frontier(self, n: int, ordered: bool = True):
-        """Resident frontier of the pieces; `ordered` lays them out largest
-        score first once, so solves need no per-call order permutation."""
+        """Resident frontier of the pieces; `ordered` lays them out in
+        `order()` once, so solves need no per-call order permutation."""
         from . import SymPieces
         if not ordered:
             return SymPieces(n, self.cycles, self.levels, self.origins, self.mults, self.num_origins)
@@ -232,7 +248,8 @@ class Pieces:
         return SymPieces(n, self.cycles[o], self.levels[o], self.origins[o], self.mults[o], self.num_origins)
 
     def order(self) -> np.ndarray:
-        return learned_order(self.score)
+        """Largest size class first, same-level siblings together (class_order)."""
+        return class_order(self.score, self.levels)
 
 
 def split_cases(n: int, level: int, cycles, mults, score_fn, threshold: float, max_extra: int = 8,

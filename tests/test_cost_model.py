@@ -70,6 +70,26 @@ def test_sort_by_score_semantics():
     assert cost.sort_by_score(scores, mults, ascending=True).tolist() == [3, 2, 5, 0, 4, 1]
 
 
+def test_class_order_semantics():
+    """Pieces: size classes of width 2 (log2 nodes) largest first, then level,
+    then the splitter's order; a permutation whatever the scores."""
+    pred = [3.9, 9.0, 2.0, 8.1, 9.5, 3.0, 8.9]
+    levels = [37, 36, 36, 38, 37, 36, 36]
+    # classes: floor(p/2) = 1, 4, 1, 4, 4, 1, 4
+    assert cost.class_order(pred, levels).tolist() == [1, 6, 4, 3, 2, 5, 0]
+    assert cost.class_order(pred, levels, width=100.0).tolist() == [1, 2, 5, 6, 0, 4, 3]
+    rng = np.random.default_rng(3)
+    p, lv = rng.normal(8, 4, 5000), rng.integers(36, 44, 5000)
+    o = cost.class_order(p, lv)
+    assert sorted(o.tolist()) == list(range(5000))
+    k = np.floor(p[o] / 2)
+    assert (np.diff(k) <= 0).all()
+    same = np.diff(k) == 0
+    assert (np.diff(lv[o])[same] >= 0).all()
+    tie = same & (np.diff(lv[o]) == 0)
+    assert (np.diff(o.astype(np.int64))[tie] > 0).all()
+
+
 def test_schedule_metrics():
     assert cost.wave_waste([4] * 128) == pytest.approx(1.0)
     assert cost.wave_waste([8] + [0] * 63 + [8] + [0] * 63) == pytest.approx(64.0)
