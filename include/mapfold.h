@@ -109,6 +109,7 @@ typedef struct {
     uint64_t queue_capacity;  /* re-split: work items the device queue can hold */
     const uint32_t *order;    /* optional host permutation of item indices (learned order);
                                  NULL = input order.  Never changes any count. */
+    uint64_t order_len;       /* entries of `order`: must equal the frontier's item count */
 } mf_solve_options;
 
 /* ---- Lunnon DFS (mf.c / gpu_folding*.cu) --------------------------------- */
@@ -151,8 +152,24 @@ int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps,
 /* Case rows: the cycle successor of each cell, one byte per cell, in rows of
  * 64 bytes for n <= 8 and 128 bytes for n = 9 (cells past the row's level
  * are zero).  The seed center is c x c cells with c = 5 for n = 9, 4 for even
- * and 3 for odd n <= 8; the environment variable MF_CENTER_N overrides it
- * (same parity as n, 3 <= c, c*c <= 25), as in sym_fold9.cpp:968-980. */
+ * and 3 for odd n <= 8, and cells are numbered by the center-out spiral
+ * (sym_fold.cpp:122-173, 704), unless mf_sym_set_numbering changed them.
+ * The library reads no environment variable. */
+
+/* Process-wide seed center and cell numbering of every sym entry point that
+ * takes n (the sym_fold9 CLI sets them from MF_CENTER_N, MF_ORDER and
+ * MF_LOCK_CENTER_N, experiments/9x9/sym_fold9.cpp:160-229,968-980).
+ *   center: 0 = the default above, else 3..5 (checked against n: same parity,
+ *           at most n; applies to n >= 6);
+ *   order:  MF_ORDER_SPIRAL, or for odd n a ring-wise renumbering;
+ *   lock_center_n: rings of radius <= (lock_center_n - 1) / 2 keep spiral
+ *           order (sym_fold9's default is 3).
+ * The mf_wasm_* entry points always use the spiral (their records are in
+ * spiral order, mfah/wasm/sym_kernel.c:149-181). */
+enum { MF_ORDER_SPIRAL = 0, MF_ORDER_REVERSE = 1, MF_ORDER_OPPOSITE = 2, MF_ORDER_AXES_FIRST = 3,
+       MF_ORDER_CORNERS_FIRST = 4 };
+int mf_sym_set_numbering(int32_t center, int32_t order, int32_t lock_center_n);
+void mf_sym_get_numbering(int32_t *center, int32_t *order, int32_t *lock_center_n);
 
 /* Seeds: deduplicated center cycles for 6 <= n <= 9 (orbit multipliers), in
  * generation order.  *raw = center cycles before dedup. */
@@ -220,7 +237,8 @@ int mf_sym_expand(int32_t n, int32_t level, const uint8_t *cycles, int64_t count
 int mf_sym_pieces_create(int32_t n, const uint8_t *cycles, const uint8_t *levels, const uint32_t *origins,
                          const uint64_t *mults, int64_t num, int64_t num_origins, mf_frontier **out);
 /* Solve a resident frontier; per_count/per_nodes are host arrays (nullable);
- * opt->order, if given, must be a permutation of the items. */
+ * opt->order, if given, must be a permutation of the items with
+ * opt->order_len equal to their number. */
 int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt,
                       uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
 /* Solve a resident Lunnon frontier and return each map's exact total. */

@@ -30,7 +30,7 @@ __all__ = [
     "device_cus", "lunnon_frontier", "lunnon_solve", "gpu_folding", "lunnon_batch", "sym_seeds",
     "sym_cases", "sym_seed_case_counts", "ALL_SEEDS", "sym_case_total", "sym_solve", "sym_fold",
     "SymFrontier", "LunnonFrontier", "LunnonBatchFrontier", "SymPieces", "sym_expand", "lunnon_frontier_size", "case_bytes",
-    "declared_symbols",
+    "declared_symbols", "sym_set_numbering", "sym_get_numbering", "sym_numbering", "SYM_ORDERS",
 ]
 
 _P = C.POINTER
@@ -69,8 +69,9 @@ class SolveOptions:
         o.graph = int(self.graph)
         keep = None
         if self.order is not None:
-            keep = np.ascontiguousarray(self.order, dtype=np.uint32)
+            keep = np.ascontiguousarray(self.order, dtype=np.uint32).reshape(-1)
             o.order = _p(keep, C.c_uint32)
+            o.order_len = keep.shape[0]
         return o, keep
 
 
@@ -162,6 +163,40 @@ def lunnon_batch(shapes, depths, options: SolveOptions | None = None):
 
 # ----------------------------------------------------------------------------- sym
 
+SYM_ORDERS = {"spiral": 0, "reverse": 1, "opposite": 2, "axes_first": 3, "corners_first": 4}
+
+
+def sym_set_numbering(center: int = 0, order: str | int = "spiral", lock_center_n: int = 3) -> None:
+    """Process-wide seed center (0 = the reference's default for n) and cell
+    numbering of the sym entry points (sym_fold9's MF_CENTER_N, MF_ORDER,
+    MF_LOCK_CENTER_N; experiments/9x9/sym_fold9.cpp:160-229,968-980).  An
+    order name the reference does not know means the spiral, as there."""
+    o = SYM_ORDERS.get(order, 0) if isinstance(order, str) else int(order)
+    check(lib().mf_sym_set_numbering(int(center), o, int(lock_center_n)), "mf_sym_set_numbering")
+
+
+def sym_get_numbering():
+    """(center, order, lock_center_n) of the process-wide numbering."""
+    c, o, lk = C.c_int32(), C.c_int32(), C.c_int32()
+    lib().mf_sym_get_numbering(C.byref(c), C.byref(o), C.byref(lk))
+    return int(c.value), int(o.value), int(lk.value)
+
+
+class sym_numbering:
+    """Context manager: the given numbering inside, the previous one after."""
+
+    def __init__(self, center: int = 0, order: str | int = "spiral", lock_center_n: int = 3):
+        self.want = (center, order, lock_center_n)
+
+    def __enter__(self):
+        self.prev = sym_get_numbering()
+        sym_set_numbering(*self.want)
+        return self
+
+    def __exit__(self, *exc):
+        sym_set_numbering(*self.prev)
+        return False
+
 
 def sym_seeds(n: int):
     """(cycles uint8 [S, case_bytes(n)], multipliers uint64 [S], raw center count)."""
@@ -247,11 +282,15 @@ def sym_fold(n: int, expanded: int = 24, start: int = 0, max_cases: int = 0,
 
 
 class _Frontier:
-    def __init__(self, handle, num):
+    def __init__(self, handle, num, items=None):
         self._h = handle
         self.num = num
+        self.items = num if items is None else items   # resident work items (what an order permutes)
 
     def solve(self, options: SolveOptions | None = None, per_item: bool = False):
+        if options is not None and options.order is not None and self.items is not None \
+                and np.asarray(options.order).size != self.items:
+            raise ValueError(f"order has {np.asarray(options.order).size} entries, the frontier {self.items} items")
         pc = np.zeros(self.num, dtype=np.uint64) if per_item else None
         pn = np.zeros(self.num, dtype=np.uint64) if per_item else None
         res = MFResult()
@@ -299,7 +338,7 @@ class LunnonBatchFrontier(_Frontier):
         h = C.c_void_p()
         check(lib().mf_lunnon_batch_frontier_create(_p(shapes, C.c_int32), _p(depths, C.c_int32),
                                                     shapes.shape[0], C.byref(h)), "mf_lunnon_batch_frontier_create")
-        super().__init__(h, 0)
+        super().__init__(h, 0, items=None)
         self.nmaps = shapes.shape[0]
 
     def solve_maps(self, options: SolveOptions | None = None):
@@ -355,5 +394,4 @@ class SymPieces(_Frontier):
         check(lib().mf_sym_pieces_create(n, _p(cycles, C.c_uint8), _p(levels, C.c_uint8), _p(origins, C.c_uint32),
                                          _p(mults, C.c_uint64), cycles.shape[0], int(num_origins), C.byref(h)),
               "mf_sym_pieces_create")
-        super().__init__(h, int(num_origins))
-        self.items = cycles.shape[0]
+        super().__init__(h, int(num_origins), items=cycles.shape[0])

@@ -50,7 +50,7 @@ class MFSolveOptions(C.Structure):
         ("schedule", C.c_int32), ("block", C.c_int32), ("waves_per_cu", C.c_int32),
         ("budget", C.c_int32), ("tail", C.c_int32), ("count_events", C.c_int32),
         ("core", C.c_int32), ("graph", C.c_int32),
-        ("queue_capacity", C.c_uint64), ("order", C.POINTER(C.c_uint32)),
+        ("queue_capacity", C.c_uint64), ("order", C.POINTER(C.c_uint32)), ("order_len", C.c_uint64),
     ]
 
 
@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
         "mf_gpu_folding": (C.c_int, [P(i32), i32, i32, P(MFSolveOptions), P(MFResult), P(i64)]),
         "mf_lunnon_batch": (C.c_int, [P(i32), P(i32), i32, P(MFSolveOptions), P(u64), P(MFResult)]),
         "mf_sym_seeds": (C.c_int, [i32, P(u8), P(u64), i64, P(i64), P(i64)]),
+        "mf_sym_set_numbering": (C.c_int, [i32, i32, i32]),
+        "mf_sym_get_numbering": (None, [P(i32), P(i32), P(i32)]),
         "mf_lunnon_check_states": (C.c_int, [P(i32), i32, i32, P(u8), i64, P(i64)]),
         "mf_sym_cases": (C.c_int, [i32, i32, u64, u64, P(u8), P(u64), P(i64)]),
         "mf_sym_case_total": (C.c_int, [i32, i32, P(u64)]),

@@ -475,6 +475,9 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
     // origins stay attached to each item so results are unchanged.
     DevBuf reordered;
     const bool ordered = opt && opt->order && f->num > 0;
+    if (opt && opt->order && opt->order_len != f->num)
+        return fail(MF_ERR_ARG, "order has %llu entries, the frontier %llu items",
+                    static_cast<unsigned long long>(opt->order_len), static_cast<unsigned long long>(f->num));
     if (f->kind == 1 || f->kind == 3) {
         const bool c96 = f->kind == 3;
         const int core = c96 ? 2 : t.core;
@@ -820,6 +823,23 @@ int mf_lunnon_batch(const int32_t *dims2, const int32_t *depths, int32_t nmaps, 
 
 // ------------------------------------------------------------------- sym
 
+int mf_sym_set_numbering(int32_t center, int32_t order, int32_t lock_center_n) {
+    SymNumbering num;
+    num.center = center;
+    num.order = order;
+    num.lock_center_n = lock_center_n;
+    std::string err;
+    if (!set_sym_numbering(num, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    return MF_OK;
+}
+
+void mf_sym_get_numbering(int32_t *center, int32_t *order, int32_t *lock_center_n) {
+    const SymNumbering num = sym_numbering();
+    if (center) *center = num.center;
+    if (order) *order = num.order;
+    if (lock_center_n) *lock_center_n = num.lock_center_n;
+}
+
 int mf_sym_seeds(int32_t n, uint8_t *cycles, uint64_t *mults, int64_t max_seeds, int64_t *num_seeds, int64_t *raw) {
     SymProblem pr;
     std::string err;
@@ -896,23 +916,27 @@ int mf_sym_case_total(int32_t n, int32_t expanded, uint64_t *total) {
     return MF_OK;
 }
 
+// num: the cell numbering (NULL = the process-wide one).
 static int sym_frontier_create_stop(int32_t n, int32_t level, int32_t stop, const uint8_t *cycles,
-                                    const uint64_t *mults, int64_t num, mf_frontier **out);
+                                    const uint64_t *mults, int64_t num, mf_frontier **out,
+                                    const SymNumbering *numbering);
 
 int mf_sym_frontier_create(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults, int64_t num,
                            mf_frontier **out) {
-    return sym_frontier_create_stop(n, level, n * n, cycles, mults, num, out);
+    return sym_frontier_create_stop(n, level, n * n, cycles, mults, num, out, nullptr);
 }
 
 static int sym_frontier_create_stop(int32_t n, int32_t level, int32_t stop, const uint8_t *cycles,
-                                    const uint64_t *mults, int64_t num, mf_frontier **out) {
+                                    const uint64_t *mults, int64_t num, mf_frontier **out,
+                                    const SymNumbering *numbering) {
     int rc = require_init();
     if (rc != MF_OK) return rc;
     if (!out || num < 0 || (num > 0 && !cycles)) return fail(MF_ERR_ARG, "bad frontier arguments");
     if (num > 0xffffffffll) return fail(MF_ERR_CAPACITY, "too many cases for one frontier");
     SymProblem pr;
     std::string err;
-    if (!sym_problem(n, pr, err)) return fail(MF_ERR_ARG, "%s", err.c_str());
+    const bool ok = numbering ? sym_problem(n, *numbering, pr, err) : sym_problem(n, pr, err);
+    if (!ok) return fail(MF_ERR_ARG, "%s", err.c_str());
     if (n < 2) return fail(MF_ERR_ARG, "n must be at least 2 for the cycle DFS");
     if (stop < 2 || stop > pr.n2) return fail(MF_ERR_DEPTH, "stop level must be between 2 and %d", pr.n2);
     if (level < 1 || level > stop - 1) return fail(MF_ERR_DEPTH, "level must be between 1 and %d", stop - 1);
@@ -1161,11 +1185,17 @@ uint32_t mf_wasm_solve_cases(uint32_t n, uint32_t start_i, uint32_t stop_i, uint
         for (uint32_t k = 0; k < case_count; ++k) total += mults[k];
         return wasm_write(total, case_count, 0);
     }
+    // Level 0 below the stop level: get_good(cycle, 0) is empty (no cell lies
+    // below cell 0), so every case is one childless node (sym_kernel.c:183,
+    // 212-230).
+    if (start_i == 0) return wasm_write(0, case_count, 0);
     // Multipliers can exceed the kernel's 16-bit field: count with 1 and
-    // weight on the host from the per-case results.
+    // weight on the host from the per-case results.  Records are in spiral
+    // order (sym_kernel.c make_edges), whatever the process-wide numbering.
     mf_frontier *f = nullptr;
+    const SymNumbering spiral_numbering;
     if (sym_frontier_create_stop(static_cast<int32_t>(n), static_cast<int32_t>(start_i), static_cast<int32_t>(stop_i),
-                                 cycles.data(), nullptr, case_count, &f) != MF_OK)
+                                 cycles.data(), nullptr, case_count, &f, &spiral_numbering) != MF_OK)
         return wasm_write(0, 0, 3);
     std::vector<uint64_t> per(case_count, 0);
     mf_result r;

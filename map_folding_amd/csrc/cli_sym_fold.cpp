@@ -154,8 +154,8 @@ void count_summary(const char *label, std::vector<unsigned __int128> v) {
               << " total=" << to_decimal(total) << "\n";
 }
 
-// Center side as choose_center_size (sym_fold9.cpp:968-980); the library
-// reads the same MF_CENTER_N.
+// Center side as choose_center_size (sym_fold9.cpp:968-980), handed to the
+// library with mf_sym_set_numbering.
 int choose_center(int n) {
     int c = n >= 9 ? 5 : ((n % 2 == 0) ? 4 : 3);
     if (const char *text = std::getenv("MF_CENTER_N")) c = std::atoi(text);
@@ -216,12 +216,6 @@ int main(int argc, char **argv) {
         usage(argv[0]);
         return 1;
     }
-    // The 8x8 tool knows no MF_CENTER_N / MF_ORDER: keep the library on its
-    // default center and the spiral numbering.
-    if (!kNine) {
-        unsetenv("MF_CENTER_N");
-        unsetenv("MF_ORDER");
-    }
     const int n = mfcli::parse_int(argv[1], "n");
     if (n < 1 || n > MF_CLI_MAX_N) {
         std::cerr << "n must be between 1 and " << MF_CLI_MAX_N << "\n";
@@ -245,6 +239,23 @@ int main(int argc, char **argv) {
     if (n == 2) {
         std::cout << "8\n";
         return 0;
+    }
+    // sym_fold9's seed center and cell numbering come from its environment
+    // (sym_fold9.cpp:160-229,968-980); the 8x8 tool knows neither and keeps
+    // the library's defaults (the spiral, c = 4 or 3).
+    if (kNine) {
+        const int32_t c = n >= 6 ? choose_center(n) : 0;
+        int32_t order = MF_ORDER_SPIRAL, lock = 3;
+        if (const char *text = std::getenv("MF_ORDER")) {
+            const std::string o = text;
+            order = o == "reverse" ? MF_ORDER_REVERSE
+                    : o == "opposite" ? MF_ORDER_OPPOSITE
+                    : o == "axes_first" ? MF_ORDER_AXES_FIRST
+                    : o == "corners_first" ? MF_ORDER_CORNERS_FIRST : MF_ORDER_SPIRAL;
+            if (const char *lt = std::getenv("MF_LOCK_CENTER_N")) lock = std::max(0, std::atoi(lt));
+        }
+        const int rc = mf_sym_set_numbering(c, order, lock);
+        if (rc != MF_OK) mfcli::die(rc, "mf_sym_set_numbering");
     }
     const int n2 = n * n;
     int expanded = 0;

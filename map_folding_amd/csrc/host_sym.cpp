@@ -105,18 +105,8 @@ void cell_map(int w, int h, int order, int lock_radius, int *map) {
     for (int k = 0; k < n * n; ++k) map[ordered[k].first * n + ordered[k].second] = k;
 }
 
-// MF_ORDER (unset, empty, "spiral" or an unknown name: the spiral) and
-// MF_LOCK_CENTER_N (default 3: rings of radius <= (3-1)/2 keep spiral order).
-void order_from_env(int &order, int &lock_radius) {
-    order = 0;
-    lock_radius = 1;
-    const char *text = std::getenv("MF_ORDER");
-    if (text == nullptr) return;
-    const std::string o = text;
-    order = o == "reverse" ? 1 : o == "opposite" ? 2 : o == "axes_first" ? 3 : o == "corners_first" ? 4 : 0;
-    const char *lock = std::getenv("MF_LOCK_CENTER_N");
-    lock_radius = std::max(0, ((lock ? std::atoi(lock) : 3) - 1) / 2);
-}
+std::mutex g_numbering_mu;
+SymNumbering g_numbering;
 
 using u128 = unsigned __int128;
 
@@ -155,7 +145,32 @@ int default_center(int n) { return n >= 9 ? 5 : ((n % 2 == 0) ? 4 : 3); }
 
 }  // namespace
 
-bool sym_problem(int n, SymProblem &pr, std::string &err) {
+SymNumbering sym_numbering() {
+    std::lock_guard<std::mutex> lock(g_numbering_mu);
+    return g_numbering;
+}
+
+bool set_sym_numbering(const SymNumbering &num, std::string &err) {
+    if (num.center != 0 && (num.center < 3 || num.center > 5)) {
+        err = "center must be 0 (default) or between 3 and 5";
+        return false;
+    }
+    if (num.order < 0 || num.order > 4) {
+        err = "order must be 0 (spiral) .. 4 (corners_first)";
+        return false;
+    }
+    if (num.lock_center_n < 0) {
+        err = "lock_center_n must not be negative";
+        return false;
+    }
+    std::lock_guard<std::mutex> lock(g_numbering_mu);
+    g_numbering = num;
+    return true;
+}
+
+bool sym_problem(int n, SymProblem &pr, std::string &err) { return sym_problem(n, sym_numbering(), pr, err); }
+
+bool sym_problem(int n, const SymNumbering &num, SymProblem &pr, std::string &err) {
     if (n < 1 || n > 9) {
         err = "n must be between 1 and 9";
         return false;
@@ -163,15 +178,16 @@ bool sym_problem(int n, SymProblem &pr, std::string &err) {
     pr.n = n;
     pr.n2 = n * n;
     pr.center = default_center(n);
-    if (const char *text = std::getenv("MF_CENTER_N")) {
-        const int c = std::atoi(text);
-        if (n >= 6 && (c < 3 || c > n || c * c > 25 || (c % 2) != (n % 2))) {
-            err = "MF_CENTER_N must be between 3 and n, have the same parity as n, and be at most 5";
+    if (num.center != 0 && n >= 6) {
+        const int c = num.center;
+        if (c < 3 || c > n || c * c > 25 || (c % 2) != (n % 2)) {
+            err = "the seed center must be between 3 and n, have the same parity as n, and be at most 5";
             return false;
         }
-        if (n >= 6) pr.center = c;
+        pr.center = c;
     }
-    order_from_env(pr.order, pr.lock_radius);
+    pr.order = num.order;
+    pr.lock_radius = std::max(0, (num.lock_center_n - 1) / 2);
     if (pr.order == 0 || n % 2 == 0) pr.order = pr.lock_radius = 0;   // numbering is the spiral
     std::memset(pr.edge, pr.n2, sizeof(pr.edge));
     cell_map(n, n, pr.order, pr.lock_radius, pr.map);

@@ -59,10 +59,26 @@ struct SymProblem {
     int case_bytes() const { return n2 <= 64 ? kCells : kMaxCells; }
 };
 
-// n in 1..9.  The seed center is c = 5 for n = 9, else 4 (n even) or 3 (n
-// odd), overridden by MF_CENTER_N as in sym_fold9.cpp (same parity as n,
-// c*c <= 25).  Cells are numbered by the center-out spiral, or for odd n by
-// MF_ORDER / MF_LOCK_CENTER_N's ring order (sym_fold9.cpp:160-229).
+// Seed center and cell numbering of the symmetry DFS.  The defaults are the
+// reference's: center c = 5 for n = 9, else 4 (n even) or 3 (n odd), and the
+// center-out spiral.  sym_fold9 lets MF_CENTER_N / MF_ORDER /
+// MF_LOCK_CENTER_N change them (sym_fold9.cpp:160-229,968-980); its CLI reads
+// those variables, the library reads no environment.
+struct SymNumbering {
+    int center = 0;          // 0 = the default for n; else 3..5, same parity as n (n >= 6)
+    int order = 0;           // 0 spiral, 1 reverse, 2 opposite, 3 axes_first, 4 corners_first
+    int lock_center_n = 3;   // rings of radius <= (lock_center_n - 1) / 2 keep spiral order
+};
+
+// The process-wide numbering (mf_sym_set_numbering); every entry point that
+// takes only n uses it.
+SymNumbering sym_numbering();
+bool set_sym_numbering(const SymNumbering &num, std::string &err);
+
+// n in 1..9 under the given numbering (an odd n only ever renumbers; the
+// center override applies to n >= 6).
+bool sym_problem(int n, const SymNumbering &num, SymProblem &pr, std::string &err);
+// The same under the process-wide numbering.
 bool sym_problem(int n, SymProblem &pr, std::string &err);
 
 // Host get_good (generators, features).  sym_good: n <= 8; sym_good128: any n.

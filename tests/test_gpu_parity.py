@@ -105,6 +105,28 @@ def test_lunnon_batch_small_rectangles(gpu, golden):
     assert res.count == sum(totals)
 
 
+def test_lunnon_batch_bench_draw_full_size(gpu, golden):
+    """The benchmark's own draw: 64 rectangles from seed 0 at full size (p in
+    {2,3,4}, 12 <= pq <= 30, so 2x15 and 3x10 are in it), each with the
+    bench's default per-map depth, in one resident batch frontier.  Every
+    per-map total must equal `mf p q`, including those above 2^32
+    (2x15 = 39,409,195,740; 3x10 = 17,257,455,960)."""
+    import bench
+    shapes = bench.rect_shapes(0)
+    assert shapes == _rect_draw(0, 64)
+    assert (2, 15) in shapes and (3, 10) in shapes
+    depths = [bench.rect_depth(p, q) for p, q in shapes]
+    _say(f"lunnon batch: the 64-rectangle draw of seed 0, depths {min(depths)}..{max(depths)}")
+    front = gpu.LunnonBatchFrontier(shapes, depths)
+    for sched in ("resplit", "resplit_nograph"):
+        totals, res = front.solve_maps(_opts(gpu, sched, count_events=False))
+        for (p, q), t in zip(shapes, totals):
+            assert t == golden["rectangles"][f"{p}x{q}"], (p, q, sched)
+        assert res.count == sum(totals)
+    assert max(totals) > 2 ** 32
+    front.close()
+
+
 # ------------------------------------------------------------------------------ sym
 
 
@@ -284,3 +306,26 @@ def test_pieces_and_orders_keep_counts(gpu):
     with pytest.raises(gpu.MFError):
         f.solve(_opts(gpu, "resplit", order=bad))
     f.close()
+
+
+def test_learned_split_7x7_window(gpu, golden):
+    """The timed path of the learned schedule: pieces split by the committed
+    cost model on the 7x7 depth-24 window [200000, +20000) count the window's
+    golden total, and each origin's pieces add up to its unsplit count
+    (sym_fold.cpp:491-527 defines the window)."""
+    from map_folding_amd import cost
+    _say("learned split: 7x7 d24 [200000,+20000) with the committed cost model")
+    cyc, mul = gpu.sym_cases(7, 24, 200000, 20000)
+    ref, pc0, pn0 = gpu.sym_solve(7, 24, cyc, mul, _opts(gpu, "resplit"), per_item=True)
+    model = cost.CostModel(seed=0)
+    model.load(os.path.join(ROOT, "map_folding_amd", "cost_default.npy"))
+    pieces = cost.learned_split(model, 7, 24, cyc, mul, 12.0)
+    assert len(pieces) > cyc.shape[0] and len(set(pieces.levels.tolist())) > 1
+    f = pieces.frontier(7)
+    for sched in ("resplit", "resplit_tight_nograph"):
+        r, pc, pn = f.solve(_opts(gpu, sched), per_item=True)
+        assert r.count * 49 == 4968634597136, sched
+        assert np.array_equal(pc, pc0), sched
+        assert int(pn.sum()) + pieces.host_nodes == ref.nodes, sched
+    f.close()
+    model.close()

@@ -75,10 +75,9 @@ def test_sym9_seeds_match_oracle(mf, oracle):
 
 @pytest.mark.parametrize("e,start,count,center", [(64, 123456, 300, None), (30, 1000, 500, None),
                                                   (20, 100, 200, 3), (26, 5000, 300, 5)])
-def test_sym9_case_stream_matches_oracle(mf, oracle, monkeypatch, e, start, count, center):
-    if center is not None:
-        monkeypatch.setenv("MF_CENTER_N", str(center))
-    c1, m1 = mf.sym_cases(9, e, start, count)
+def test_sym9_case_stream_matches_oracle(mf, oracle, e, start, count, center):
+    with mf.sym_numbering(center=center or 0):
+        c1, m1 = mf.sym_cases(9, e, start, count)
     c2, m2 = oracle.sym9_cases(9, e, start, count, c=center)
     assert c1.shape == c2.shape == (count, 128)
     assert np.array_equal(c1, c2) and np.array_equal(m1, m2)
@@ -226,18 +225,45 @@ def test_sym_cell_orders_match_oracle(mf, oracle, monkeypatch, order, lock):
     product seeds and case windows equal the oracle's under each numbering
     (7x7 with its 3x3 center, and 9x9 with the 5x5 center whose symmetry maps
     follow the renumbered center)."""
+    # The oracle reads the reference's environment variables; the library
+    # takes the numbering explicitly.
     monkeypatch.setenv("MF_ORDER", order)
     if lock:
         monkeypatch.setenv("MF_LOCK_CENTER_N", lock)
     cases = [(7, 26, 3000, 400)] + ([(9, 62, 1000, 300)] if order == "reverse" else [])
     for n, e, start, count in cases:
-        c1, m1, r1 = mf.sym_seeds(n)
+        with mf.sym_numbering(order=order, lock_center_n=int(lock) if lock else 3):
+            c1, m1, r1 = mf.sym_seeds(n)
+            k1, w1 = mf.sym_cases(n, e, start, count)
         c2, m2, r2 = oracle.sym9_seeds(n)
         assert r1 == r2 and np.array_equal(c1[:, :c2.shape[1]] if n < 9 else c1, c2[:, :c1.shape[1]])
         assert np.array_equal(m1, m2)
-        k1, w1 = mf.sym_cases(n, e, start, count)
         k2, w2 = oracle.sym9_cases(n, e, start, count)
         assert np.array_equal(k1, k2[:, :k1.shape[1]]) and np.array_equal(w1, w2)
+
+
+def test_library_ignores_numbering_environment(mf, oracle, monkeypatch):
+    """The library reads no MF_ORDER / MF_CENTER_N: with them set, the 7x7
+    case stream and seeds are still the spiral ones of sym_fold.cpp (only the
+    sym_fold9 CLI passes them on, through mf_sym_set_numbering)."""
+    want_c, want_m = oracle.sym_cases(7, 26, 3000, 200)
+    monkeypatch.setenv("MF_ORDER", "reverse")
+    monkeypatch.setenv("MF_CENTER_N", "5")
+    c1, m1 = mf.sym_cases(7, 26, 3000, 200)
+    assert np.array_equal(c1, want_c) and np.array_equal(m1, want_m)
+    assert mf.sym_get_numbering() == (0, 0, 3)
+
+
+def test_numbering_setting_is_checked(mf):
+    with pytest.raises(mf.MFError):
+        mf.sym_set_numbering(center=7)
+    with pytest.raises(mf.MFError):
+        mf.sym_set_numbering(order=9)
+    # a center of the wrong parity is refused where n is known
+    with mf.sym_numbering(center=4):
+        with pytest.raises(mf.MFError):
+            mf.sym_seeds(7)
+    assert mf.sym_get_numbering() == (0, 0, 3)
 
 
 @pytest.mark.parametrize("dims,depth", [([2, 3], 4), ([3, 3], 6), ([2, 4], 6), ([3, 4], 7), ([2, 2, 2], 6)])

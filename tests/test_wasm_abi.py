@@ -64,3 +64,32 @@ def test_wasm_payloads_match_oracle(gpu, oracle, n, depth, stop, k):
     status = _lib(gpu).mf_wasm_solve_cases(n, depth, stop, len(chosen))
     assert status == 0
     assert _read_out(gpu) == (want_c, want_n, 0)
+
+
+def test_wasm_level_zero_is_one_childless_node_per_case(mf):
+    """start_i = 0 < stop_i: get_good(cycle, 0) is empty, so sym_kernel.c
+    counts 0 with one node per case and status 0 (sym_kernel.c:183,212-230,
+    263-276).  No kernel runs for it."""
+    L = _lib(mf)
+    _write_cases(mf, [np.zeros(64, dtype=np.uint8)] * 5, [3] * 5, 6)
+    assert L.mf_wasm_solve_cases(6, 0, 20, 5) == 0
+    assert _read_out(mf) == (0, 5, 0)
+
+
+@pytest.mark.gpu
+def test_wasm_records_stay_spiral_under_another_numbering(gpu, oracle, monkeypatch):
+    """The wasm records are in spiral order whatever the process-wide
+    numbering (and the environment) say: a 7x7 payload under MF_ORDER=reverse
+    and a reverse library numbering still equals the spiral oracle."""
+    prefixes = oracle.sym_prefixes(7, 14)
+    chosen = prefixes[:: max(1, prefixes.shape[0] // 32)][:32]
+    want_c, want_n = 0, 0
+    for cyc in chosen:
+        c, nodes = oracle.sym_count_to(7, cyc, 14, 28)
+        want_c += c
+        want_n += nodes
+    monkeypatch.setenv("MF_ORDER", "reverse")
+    with gpu.sym_numbering(order="reverse"):
+        _write_cases(gpu, chosen, [1] * len(chosen), 7)
+        assert _lib(gpu).mf_wasm_solve_cases(7, 14, 28, len(chosen)) == 0
+    assert _read_out(gpu) == (want_c, want_n, 0)
