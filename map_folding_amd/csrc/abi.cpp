@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/mapfold.h"
+#include "mf_counters.hpp"
 #include "mf_device.hpp"
 #include "mf_host.hpp"
 
@@ -128,6 +129,8 @@ Tuned tune(const mf_solve_options *opt) {
     t.waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
     // A step expands one node (or recomputes one).
     t.budget = (opt && opt->budget > 0) ? static_cast<uint32_t>(opt->budget) : 64u;
+    // Per-item event counters are folded into 64 bits every `budget` steps.
+    t.budget = std::min(t.budget, kMaxFoldBudget);
     t.tail = (opt && opt->tail > 0) ? static_cast<uint32_t>(opt->tail) : t.budget;
     t.events = opt ? opt->count_events != 0 : true;
     t.capacity = (opt && opt->queue_capacity > 0) ? opt->queue_capacity : 0;
@@ -580,8 +583,28 @@ WorkItem lunnon_item(const uint8_t *b, int depth, int n, uint32_t origin, int ma
 
 }  // namespace
 
-// Shared with the other translation units of the library (cost model).
+// Shared with the other translation units of the library (cost model,
+// the no-allocation solve).
 namespace mf {
+bool valid_case_cycle(const uint8_t *cycle, int level) { return valid_cycle(cycle, level); }
+WorkItem make_sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
+    return sym_item(cycle, mult, level, origin);
+}
+WorkItem128 make_sym_item128(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
+    return sym_item128(cycle, mult, level, origin);
+}
+// The initialised context as it is (never initialises: mf_init allocates).
+bool lib_current(LibContext *c) {
+    if (g.device < 0) return false;
+    c->device = g.device;
+    c->cus = g.cus;
+    c->stream = g.stream;
+    c->ev0 = g.ev0;
+    c->ev1 = g.ev1;
+    c->host_acc = g.host_acc;
+    c->host_counters = g.host_counters;
+    return true;
+}
 int lib_error(int status, const char *msg) { return fail(status, "%s", msg); }
 int lib_context(hipStream_t *stream, int *cus) {
     const int rc = require_init();

@@ -23,6 +23,7 @@
 // inserts and completions do not depend on it.
 #include <hip/hip_runtime.h>
 
+#include "mf_counters.hpp"
 #include "mf_device.hpp"
 #include "mf_dfs.hpp"
 #include "mf_wave.hpp"
@@ -44,7 +45,7 @@ struct LWork {
     uint32_t floor;   // 0, or j+1: recompute the gaps of leaf l, keep those above j
     uint64_t comps;   // completions of the current item
     uint32_t inodes;  // inserts, gap computations, walk steps, candidates of the item
-    uint32_t iwalks;
+    uint32_t iwalks;  // (32-bit: folded into the lane totals before they can wrap, lfold_item)
     uint32_t isteps;
     uint32_t icands;
     bool have;
@@ -225,6 +226,19 @@ __device__ __forceinline__ void lretire(const LunnonArgs &a, const LWork &w, LTo
     if (a.per_map != nullptr && v != 0) atomicAdd(a.per_map + w.map, v);
 }
 
+// Fold the item's 32-bit event counters into the lane totals once one of
+// them reaches kFoldAt (mf_counters.hpp).
+__device__ __forceinline__ void lfold_item(const LunnonArgs &a, LWork &w, LTotals &t) {
+    if (((w.inodes | w.iwalks | w.isteps | w.icands) & kFoldAt) == 0u) return;
+    FoldTotals f;
+    const uint32_t moved = fold_item_counters(w.inodes, w.iwalks, w.isteps, w.icands, f);
+    t.nodes += f.nodes;
+    t.walks += f.walks;
+    t.steps += f.steps;
+    t.cands += f.cands;
+    if (a.per_nodes != nullptr && moved != 0) atomicAdd(a.per_nodes + w.origin, static_cast<unsigned long long>(moved));
+}
+
 __device__ __forceinline__ void lflush(const LunnonArgs &a, uint32_t lane, const LTotals &t) {
     add_u128(a.acc, t.sum);
     const unsigned long long n = wave_sum(t.nodes), wk = wave_sum(t.walks), st = wave_sum(t.steps),
@@ -342,6 +356,8 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
         if (ldfs_step<Stack, EV>(lds, ln, w)) {
             lretire(a, w, tot);
             w.have = false;
+        } else {
+            lfold_item(a, w, tot);
         }
     }
     lflush(a, threadIdx.x & 63u, tot);
@@ -402,6 +418,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                 w.have = false;
             }
         }
+        if (w.have) lfold_item(a, w, tot);
 
         if (drained) {
             tail_left = tail_left > a.budget ? tail_left - a.budget : 0;

@@ -11,6 +11,11 @@
 #include <string>
 #include <vector>
 
+// HIP's stream and event handles (hipStream_t, hipEvent_t) without the HIP
+// headers, which the g++-built host sources do not include.
+struct ihipStream_t;
+struct ihipEvent_t;
+
 namespace mf {
 
 constexpr int kMaxLeaves = 64;   // n = prod(dims) <= 64 (one u64 mask per level)
@@ -110,5 +115,16 @@ std::vector<uint64_t> sym_slice_case_counts(const SymProblem &pr, int expanded, 
 // number written.
 int64_t sym_cases(const SymProblem &pr, int expanded, uint64_t start, uint64_t max_cases,
                   uint8_t *cycles, uint64_t *mults, uint64_t seed_start = 0, uint64_t seed_count = ~0ull);
+
+// The library context as seen by its other translation units.
+struct LibContext {
+    int device = -1;
+    int cus = 0;
+    ::ihipStream_t *stream = nullptr;
+    ::ihipEvent_t *ev0 = nullptr, *ev1 = nullptr;
+    unsigned long long *host_acc = nullptr;   // pinned, ACC_SLOTS entries
+    uint32_t *host_counters = nullptr;        // pinned, 4 entries
+};
+bool lib_current(LibContext *c);
 
 }  // namespace mf

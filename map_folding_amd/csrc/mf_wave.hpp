@@ -39,6 +39,36 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want, uin
     return base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
 }
 
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+// Reserve the wave's spill slots in one step: every lane passes how many
+// items it will write; lane 0 adds the wave's total to *counter with a
+// compare-and-swap unless that would pass `cap`.  Returns, in every lane, the
+// first of the wave's `total` consecutive slots, or kNoSlot in every lane if
+// the queue cannot take them all (then nothing is reserved).  A wave with
+// nothing to write reserves nothing and gets 0.  Must be called by the whole
+// wave (converged).
+__device__ __forceinline__ uint32_t spill_reserve(uint32_t *counter, uint32_t cap, uint32_t need, uint32_t lane) {
+    uint32_t total = need;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off, 64);
+    if (total == 0) return 0;
+    uint32_t base = kNoSlot;
+    if (lane == 0) {
+        uint32_t old = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            if (old > cap || total > cap - old) break;
+            const uint32_t prev = atomicCAS(counter, old, old + total);
+            if (prev == old) {
+                base = old;
+                break;
+            }
+            old = prev;
+        }
+    }
+    return __shfl(base, 0, 64);
+}
+
 // Copy 64 state bytes between an item and a lane's lane-interleaved LDS rows
 // (dword w of the lane at lane_base + w*256).
 __device__ __forceinline__ void state_to_lds(uint8_t *lds, uint32_t lane_base, const uint8_t *src) {

@@ -41,6 +41,7 @@
 // Counts never depend on encoding or schedule: every subtree is counted once.
 #include <hip/hip_runtime.h>
 
+#include "mf_counters.hpp"
 #include "mf_device.hpp"
 #include "mf_dfs.hpp"
 #include "mf_wave.hpp"
@@ -157,8 +158,8 @@ struct Work {
                        // keeps only children above j (not counted as a node)
     uint64_t leaves;   // leaves of the current item so far
     uint32_t inodes;   // nodes, page walks and walk steps of the current item
-    uint32_t iwalks;
-    uint32_t isteps;
+    uint32_t iwalks;   // (32-bit: folded into the lane totals before they can
+    uint32_t isteps;   // wrap, fold_item below)
     bool have;
     bool popping;      // backtracking continues this step (no node to expand)
 };
@@ -450,6 +451,20 @@ __device__ __forceinline__ void retire(const SymArgs &a, const Work &w, Totals &
     if (a.per_nodes != nullptr && w.inodes != 0) atomicAdd(a.per_nodes + w.origin, static_cast<unsigned long long>(w.inodes));
 }
 
+// Fold the item's 32-bit event counters into the lane totals once one of
+// them reaches kFoldAt (mf_counters.hpp): called after every step of the
+// static schedule and every `budget` steps of the persistent one.
+__device__ __forceinline__ void fold_item(const SymArgs &a, Work &w, Totals &t) {
+    if (((w.inodes | w.iwalks | w.isteps) & kFoldAt) == 0u) return;
+    FoldTotals f;
+    uint32_t no_cands = 0;
+    const uint32_t moved = fold_item_counters(w.inodes, w.iwalks, w.isteps, no_cands, f);
+    t.nodes += f.nodes;
+    t.walks += f.walks;
+    t.steps += f.steps;
+    if (a.per_nodes != nullptr && moved != 0) atomicAdd(a.per_nodes + w.origin, static_cast<unsigned long long>(moved));
+}
+
 __device__ __forceinline__ void flush_wave(const SymArgs &a, uint32_t lane, const Totals &t) {
     add_u128(a.acc, t.sum);
     const unsigned long long n = wave_sum(t.nodes), wk = wave_sum(t.walks), st = wave_sum(t.steps),
@@ -499,36 +514,45 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
 // Split everything the lane holds into the spill queue: the node about to be
 // expanded, then for each stack level, deepest first, the siblings not yet
 // entered (a recorded sibling as a mask, unrecorded ones as a recompute piece
-// with a floor).  Called by the whole wave.
+// with a floor).  Called by the whole wave.  The wave first reserves all its
+// slots with one compare-and-swap (spill_reserve); if the queue cannot take
+// them, nothing is written and the wave keeps its work (returns false), so
+// the queue never overflows whatever its capacity.
 template <typename G>
-__device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const Lane &ln, uint32_t lane, Work &w,
+__device__ __forceinline__ bool spill_all(const SymArgs &a, uint8_t *lds, const Lane &ln, uint32_t lane, Work &w,
                                           Totals &tot) {
     using Stack = typename G::Stack;
     using Item = typename G::Item;
+    // Items this lane emits: the pending node (unless it is mid-backtrack,
+    // when levels >= i are finished), then every level with siblings left.
+    uint32_t need = 0;
+    if (w.have) {
+        need = w.popping ? 0u : 1u;
+        for (uint32_t t = w.root; t < w.i; ++t) need += Stack::more(Stack::get(lds, ln, t)) != 0u ? 1u : 0u;
+    }
+    // `off` is wave-uniform: the next free slot of the wave's reserved range.
+    // Each pass writes the emitting lanes' items at consecutive slots in lane
+    // order; the passes together write exactly `total` items.
+    uint32_t off = spill_reserve(a.spill_count, a.spill_cap, need, lane);
+    if (off == kNoSlot) return false;
     bool pend = w.have;
     uint32_t t = w.i;
     typename G::Mask mask = 0;
     uint32_t floor = w.floor;   // the pending expansion may itself be a recompute
-    // A lane in the middle of backtracking has no pending node: levels >= i
-    // are finished, so it only continues the unwind, emitting siblings.
     bool emit = pend && !w.popping;
+    const unsigned long long below_lane = (1ull << lane) - 1ull;
     while (__any(pend)) {
-        uint32_t cnt = 0;
-        const uint32_t slot = wave_alloc(a.spill_count, emit, lane, &cnt);
+        const unsigned long long m = __ballot(emit);
         if (emit) {
-            if (slot < a.spill_cap) {
-                Item *dst = static_cast<Item *>(a.spill) + slot;
-                row_store<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, dst->state);
-                G::set_item_mask(dst, mask);
-                dst->origin = w.origin;
-                dst->mult = static_cast<uint16_t>(w.mult);
-                dst->level = static_cast<uint8_t>(t);
-                dst->map = static_cast<uint8_t>(floor);
-            } else {
-                atomicOr(a.acc + ACC_ERROR, 1ull);
-            }
+            Item *dst = static_cast<Item *>(a.spill) + off + static_cast<uint32_t>(__popcll(m & below_lane));
+            row_store<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, dst->state);
+            G::set_item_mask(dst, mask);
+            dst->origin = w.origin;
+            dst->mult = static_cast<uint16_t>(w.mult);
+            dst->level = static_cast<uint8_t>(t);
+            dst->map = static_cast<uint8_t>(floor);
         }
-        if (cnt != 0 && lane == 0) atomicAdd(a.acc + ACC_SPILLED, static_cast<unsigned long long>(cnt));
+        off += static_cast<uint32_t>(__popcll(m));
         emit = false;
         if (pend) {
             if (t == w.root) {
@@ -554,6 +578,7 @@ __device__ __forceinline__ void spill_all(const SymArgs &a, uint8_t *lds, const 
         retire(a, w, tot);
         w.have = false;
     }
+    return true;
 }
 
 }  // namespace
@@ -575,6 +600,8 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
         if (dfs_step<G, EV>(lds, ln, a.last, w)) {
             retire(a, w, tot);
             w.have = false;
+        } else {
+            fold_item(a, w, tot);
         }
     }
     flush_wave(a, threadIdx.x & 63u, tot);
@@ -638,6 +665,7 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                 w.have = false;
             }
         }
+        if (w.have) fold_item(a, w, tot);
 
         if (drained) {
             tail_left = tail_left > a.budget ? tail_left - a.budget : 0;
@@ -650,8 +678,9 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
                         w.have = false;
                     }
                 }
-                spill_all<G>(a, lds, ln, lane, w, tot);
-                break;
+                if (spill_all<G>(a, lds, ln, lane, w, tot)) break;
+                // The spill queue is full: the wave runs its work to the end
+                // in this round, testing again every `budget` steps.
             }
         }
     }
