@@ -215,6 +215,20 @@ int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t
                       uint64_t start, uint64_t max_cases, const mf_solve_options *opt,
                       uint64_t answer[2], mf_result *out);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same solve as mf_sym_solve (re-split schedule,
+ * host-driven rounds), but every device buffer beyond the caller's host
+ * inputs and outputs lives in one static 64 MiB device array per GPU, whose
+ * address mf_init looks up; the call allocates nothing.  Limits: at most
+ * MF_NOALLOC_MAX_CASES cases; no opt->order; mf_init must have been called
+ * (else MF_ERR_NO_DEVICE).  Calls that overlap on one GPU (several host
+ * threads) race on the shared array. */
+#define MF_NOALLOC_MAX_CASES 65536
+int mf_sym_solve_noalloc(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults,
+                         int64_t num, const mf_solve_options *opt,
+                         uint64_t *per_count, uint64_t *per_nodes, mf_result *out);
+
 /* ---- device-resident frontiers (inputs stay in HBM across solves) ------- */
 typedef struct mf_frontier mf_frontier;
 int mf_sym_frontier_create(int32_t n, int32_t level, const uint8_t *cycles, const uint64_t *mults,

@@ -109,6 +109,8 @@ def lib() -> C.CDLL:
         "mf_sym_solve": (C.c_int, [i32, i32, P(u8), P(u64), i64, P(MFSolveOptions), P(u64), P(u64),
                                    P(MFResult)]),
         "mf_sym_fold": (C.c_int, [i32, i32, u64, u64, P(MFSolveOptions), P(u64), P(MFResult)]),
+        "mf_sym_solve_noalloc": (C.c_int, [i32, i32, P(u8), P(u64), i64, P(MFSolveOptions), P(u64), P(u64),
+                                           P(MFResult)]),
         "mf_sym_frontier_create": (C.c_int, [i32, i32, P(u8), P(u64), i64, P(C.c_void_p)]),
         "mf_lunnon_frontier_create": (C.c_int, [P(i32), i32, i32, P(u8), i64, P(C.c_void_p)]),
         "mf_frontier_solve": (C.c_int, [C.c_void_p, P(MFSolveOptions), P(u64), P(u64), P(MFResult)]),

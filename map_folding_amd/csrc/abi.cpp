@@ -586,6 +586,7 @@ WorkItem lunnon_item(const uint8_t *b, int depth, int n, uint32_t origin, int ma
 // Shared with the other translation units of the library (cost model,
 // the no-allocation solve).
 namespace mf {
+int noalloc_setup(int device);
 bool valid_case_cycle(const uint8_t *cycle, int level) { return valid_cycle(cycle, level); }
 WorkItem make_sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin) {
     return sym_item(cycle, mult, level, origin);
@@ -650,6 +651,9 @@ int mf_init(int device) {
     HIP_TRY(hipEventCreate(&g.lb));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_acc), ACC_SLOTS * sizeof(unsigned long long)));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&g.host_counters), 4 * sizeof(uint32_t)));
+    // The device address of the no-allocation solve's static scratch array.
+    const int rc = noalloc_setup(device);
+    if (rc != MF_OK) return rc;
     g.device = device;
     return MF_OK;
 }
