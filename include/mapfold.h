@@ -110,6 +110,14 @@ typedef struct {
     const uint32_t *order;    /* optional host permutation of item indices (learned order);
                                  NULL = input order.  Never changes any count. */
     uint64_t order_len;       /* entries of `order`: must equal the frontier's item count */
+    int32_t deepen;           /* sym re-split, up to 8x8: before the rounds, expand the frontier on
+                                 the device until every item reaches this level (fewer stack levels
+                                 per lane, more resident waves); 0 = default (46 for 8x8, else off),
+                                 -1 = off.  Every expanded node is counted as the DFS counts it. */
+    const uint8_t *deepen_levels;  /* optional per-item deepening targets (e.g. chosen by the cost
+                                 model): item k is expanded to at least max(deepen_levels[k], the
+                                 `deepen` target); entries past n*n - 2 are clipped.  NULL = none. */
+    uint64_t deepen_levels_len;  /* entries of deepen_levels: must equal the frontier's item count */
 } mf_solve_options;
 
 /* ---- Lunnon DFS (mf.c / gpu_folding*.cu) --------------------------------- */

@@ -55,6 +55,8 @@ class SolveOptions:
     queue_capacity: int = 0
     order: np.ndarray | None = None   # permutation of item indices (learned order)
     graph: int = 0                    # re-split rounds as a HIP graph: 0 default (on), 1 on, 2 off
+    deepen: int = 0                   # sym re-split: device deepening target level: 0 default, -1 off
+    deepen_levels: np.ndarray | None = None   # per-item deepening targets (at least `deepen`)
 
     def to_c(self):
         o = MFSolveOptions()
@@ -67,11 +69,18 @@ class SolveOptions:
         o.core = int(self.core)
         o.queue_capacity = int(self.queue_capacity)
         o.graph = int(self.graph)
-        keep = None
+        o.deepen = int(self.deepen)
+        keep = []
         if self.order is not None:
-            keep = np.ascontiguousarray(self.order, dtype=np.uint32).reshape(-1)
-            o.order = _p(keep, C.c_uint32)
-            o.order_len = keep.shape[0]
+            a = np.ascontiguousarray(self.order, dtype=np.uint32).reshape(-1)
+            o.order = _p(a, C.c_uint32)
+            o.order_len = a.shape[0]
+            keep.append(a)
+        if self.deepen_levels is not None:
+            d = np.ascontiguousarray(self.deepen_levels, dtype=np.uint8).reshape(-1)
+            o.deepen_levels = _p(d, C.c_uint8)
+            o.deepen_levels_len = d.shape[0]
+            keep.append(d)
         return o, keep
 
 

@@ -51,6 +51,7 @@ class MFSolveOptions(C.Structure):
         ("budget", C.c_int32), ("tail", C.c_int32), ("count_events", C.c_int32),
         ("core", C.c_int32), ("graph", C.c_int32),
         ("queue_capacity", C.c_uint64), ("order", C.POINTER(C.c_uint32)), ("order_len", C.c_uint64),
+        ("deepen", C.c_int32), ("deepen_levels", C.POINTER(C.c_uint8)), ("deepen_levels_len", C.c_uint64),
     ]
 
 

@@ -31,6 +31,7 @@ hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows);
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
+const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
 }  // namespace mf
 
 using namespace mf;
@@ -83,6 +84,8 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;   // span of a solve
     hipEvent_t la = nullptr, lb = nullptr;     // one launch
     DevBuf items, spill_a, spill_b, acc, counters, per_count, per_nodes, per_map, tables;
+    DevBuf deep;                               // deepened frontier (sym)
+    DevBuf targets;                            // per-item deepening targets
     unsigned long long *host_acc = nullptr;   // pinned
     uint32_t *host_counters = nullptr;        // pinned: [claim, count of spill_a, count of spill_b]
     // Captured re-split rounds, reused while the launch configuration holds.
@@ -116,6 +119,7 @@ struct Tuned {
     uint32_t tail;
     bool events;
     uint64_t capacity;
+    int deepen;   // sym re-split: target level of the device deepening (0 default, -1 off)
 };
 
 constexpr int kDefaultCore = 2;   // compact 16-bit stack
@@ -135,6 +139,7 @@ Tuned tune(const mf_solve_options *opt) {
     t.events = opt ? opt->count_events != 0 : true;
     t.capacity = (opt && opt->queue_capacity > 0) ? opt->queue_capacity : 0;
     t.graph = !(opt && opt->graph == 2);
+    t.deepen = opt ? opt->deepen : 0;
     return t;
 }
 
@@ -170,22 +175,24 @@ void fill_result(mf_result *out, const unsigned long long *acc) {
 template <typename Item, typename Args, typename Launch>
 int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds_bytes, const Tuned &t,
               const Item *dev_items, uint64_t num, uint64_t origins, uint64_t *per_count, uint64_t *per_nodes,
-              uint64_t *per_map, uint32_t nmaps, mf_result *out, Launch launch) {
+              uint64_t *per_map, uint32_t nmaps, mf_result *out, bool keep_acc, Launch launch) {
     const auto t0 = std::chrono::steady_clock::now();
     std::memset(out, 0, sizeof(*out));
     int rc;
     if ((rc = g.acc.ensure(ACC_SLOTS * sizeof(unsigned long long))) != MF_OK) return rc;
     if ((rc = g.counters.ensure(4 * sizeof(uint32_t))) != MF_OK) return rc;
-    HIP_TRY(hipMemsetAsync(g.acc.ptr, 0, ACC_SLOTS * sizeof(unsigned long long), g.stream));
+    // keep_acc: a deepening pass has already counted into the accumulators
+    // and the per-origin arrays (cleared by it).
+    if (!keep_acc) HIP_TRY(hipMemsetAsync(g.acc.ptr, 0, ACC_SLOTS * sizeof(unsigned long long), g.stream));
     unsigned long long *dcount = nullptr, *dnodes = nullptr, *dmap = nullptr;
     if (per_count) {
         if ((rc = g.per_count.ensure(origins * 8)) != MF_OK) return rc;
-        HIP_TRY(hipMemsetAsync(g.per_count.ptr, 0, origins * 8, g.stream));
+        if (!keep_acc) HIP_TRY(hipMemsetAsync(g.per_count.ptr, 0, origins * 8, g.stream));
         dcount = static_cast<unsigned long long *>(g.per_count.ptr);
     }
     if (per_nodes) {
         if ((rc = g.per_nodes.ensure(origins * 8)) != MF_OK) return rc;
-        HIP_TRY(hipMemsetAsync(g.per_nodes.ptr, 0, origins * 8, g.stream));
+        if (!keep_acc) HIP_TRY(hipMemsetAsync(g.per_nodes.ptr, 0, origins * 8, g.stream));
         dnodes = static_cast<unsigned long long *>(g.per_nodes.ptr);
     }
     if (per_map) {
@@ -469,6 +476,97 @@ int reorder_items(const std::vector<Item> &src, const uint32_t *order, DevBuf &d
     return MF_OK;
 }
 
+// Default deepening target of a sym frontier (0 = none): 8x8 frontiers are
+// expanded to level 46, which leaves 17 stack levels per lane, so that six
+// 4-wave workgroups fit the LDS of a CU instead of five at depth 36 (8x8
+// chunk of 50,000 depth-36 cases: 908 ms at 46, 912 at 47, 917 at 48, 988 at
+// 45, where the block is just too large for six, 1001 without).
+int default_deepen(const mf_frontier &f) { return f.sym.n2 == 64 ? 46 : 0; }
+
+constexpr int kDeepenFull = 100;   // internal: a deepening pass ran out of room
+
+// Expand the frontier's expand items on the device until each reaches level
+// `target` (sym_deepen_kernel, one launch per level), counting every expanded
+// node into the accumulators and per-origin node array, which it clears
+// first.  The deepened items land in g.deep.  Returns kDeepenFull if the
+// buffers cannot hold a level (the caller then solves the frontier as it is).
+int deepen_frontier(const SymArgs &base, int core, const Tuned &t, uint32_t cells, const WorkItem *in,
+                    uint64_t num, uint64_t origins, bool want_count, bool want_nodes, uint32_t target,
+                    const uint8_t *item_targets, const WorkItem **out_items, uint64_t *out_num, double *ms) {
+    int rc;
+    const void *kd = sym_deepen_ptr(core, t.events, cells);
+    if (kd == nullptr) return kDeepenFull;
+    if ((rc = g.acc.ensure(ACC_SLOTS * sizeof(unsigned long long))) != MF_OK) return rc;
+    if ((rc = g.counters.ensure(4 * sizeof(uint32_t))) != MF_OK) return rc;
+    HIP_TRY(hipMemsetAsync(g.acc.ptr, 0, ACC_SLOTS * sizeof(unsigned long long), g.stream));
+    SymArgs a = base;
+    a.acc = static_cast<unsigned long long *>(g.acc.ptr);
+    a.per_count = nullptr;
+    a.per_nodes = nullptr;
+    if (want_count) {
+        if ((rc = g.per_count.ensure(origins * 8)) != MF_OK) return rc;
+        HIP_TRY(hipMemsetAsync(g.per_count.ptr, 0, origins * 8, g.stream));
+    }
+    if (want_nodes) {
+        if ((rc = g.per_nodes.ensure(origins * 8)) != MF_OK) return rc;
+        HIP_TRY(hipMemsetAsync(g.per_nodes.ptr, 0, origins * 8, g.stream));
+        a.per_nodes = static_cast<unsigned long long *>(g.per_nodes.ptr);
+    }
+    // Room for the deepened frontier and for one level in flight (the spill
+    // queues, which the rounds reuse afterwards).
+    const uint64_t cap = std::min<uint64_t>(0xffffffffull, std::max<uint64_t>(num * 8, uint64_t{24} << 20));
+    if ((rc = g.deep.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
+    if ((rc = g.spill_a.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
+    if ((rc = g.spill_b.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
+    uint32_t *counts = static_cast<uint32_t *>(g.counters.ptr);   // [1], [2]: next levels, [3]: final
+    WorkItem *bufs[2] = {static_cast<WorkItem *>(g.spill_a.ptr), static_cast<WorkItem *>(g.spill_b.ptr)};
+    const uint32_t lds = sym_lds_bytes(core, 1, 256, cells);
+    HIP_TRY(hipFuncSetAttribute(kd, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    a.final_out = g.deep.ptr;
+    a.final_count = counts + 3;
+    a.final_cap = static_cast<uint32_t>(cap);
+    a.spill_cap = static_cast<uint32_t>(cap);
+    a.target = target;
+    a.targets = nullptr;
+    if (item_targets != nullptr) {
+        if ((rc = g.targets.ensure(std::max<uint64_t>(num, 1))) != MF_OK) return rc;
+        HIP_TRY(hipMemcpyAsync(g.targets.ptr, item_targets, num, hipMemcpyHostToDevice, g.stream));
+        a.targets = static_cast<const uint8_t *>(g.targets.ptr);
+    }
+    HIP_TRY(hipMemsetAsync(counts + 3, 0, sizeof(uint32_t), g.stream));
+    HIP_TRY(hipEventRecord(g.la, g.stream));
+    const WorkItem *cur = in;
+    uint64_t cur_num = num;
+    int next = 0;
+    for (uint32_t pass = 0; cur_num > 0; ++pass) {
+        if (pass > 128) return fail(MF_ERR_INTERNAL, "deepening did not converge");
+        HIP_TRY(hipMemsetAsync(counts + 1 + next, 0, sizeof(uint32_t), g.stream));
+        a.items = cur;
+        a.num_items = static_cast<uint32_t>(cur_num);
+        a.spill = bufs[next];
+        a.spill_count = counts + 1 + next;
+        a.deep_pass = pass == 0 ? 0u : 1u;
+        void *args[] = {&a};
+        HIP_TRY(hipLaunchKernel(kd, dim3(static_cast<uint32_t>((cur_num + 255) / 256)), dim3(256), args, lds, g.stream));
+        HIP_TRY(hipMemcpyAsync(g.host_counters, counts, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipMemcpyAsync(g.host_acc + ACC_ERROR, a.acc + ACC_ERROR, 8, hipMemcpyDeviceToHost, g.stream));
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        if (g.host_acc[ACC_ERROR] & 6ull) return fail(MF_ERR_INTERNAL, "device consistency check failed");
+        if (g.host_acc[ACC_ERROR] & kErrDeepenFull) return kDeepenFull;
+        cur = bufs[next];
+        cur_num = g.host_counters[1 + next];
+        next ^= 1;
+    }
+    HIP_TRY(hipEventRecord(g.lb, g.stream));
+    HIP_TRY(hipEventSynchronize(g.lb));
+    float one = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&one, g.la, g.lb));
+    *ms = one;
+    *out_items = static_cast<const WorkItem *>(g.deep.ptr);
+    *out_num = g.host_counters[3];
+    return MF_OK;
+}
+
 int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_count, uint64_t *per_nodes,
                    uint64_t *per_map, mf_result *out) {
     int rc = require_init();
@@ -497,11 +595,66 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         if (c96) {
             const WorkItem128 *dev = static_cast<const WorkItem128 *>(f->items.ptr);
             if (ordered && (rc = reorder_items(f->host_items128, opt->order, reordered, &dev)) != MF_OK) return rc;
-            rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, launch);
+            rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, false, launch);
         } else {
             const WorkItem *dev = static_cast<const WorkItem *>(f->items.ptr);
             if (ordered && (rc = reorder_items(f->host_items, opt->order, reordered, &dev)) != MF_OK) return rc;
-            rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, launch);
+            // Re-split schedule: deepen the frontier on the device first, so
+            // that a lane needs fewer stack levels and more waves fit a CU.
+            int target = t.deepen == 0 ? default_deepen(*f) : t.deepen;
+            // Per-item targets (in the order the items are solved), at least
+            // the uniform one, at most n*n - 2.
+            std::vector<uint8_t> item_targets;
+            if (opt && opt->deepen_levels) {
+                if (opt->deepen_levels_len != f->num)
+                    return fail(MF_ERR_ARG, "deepen_levels has %llu entries, the frontier %llu items",
+                                static_cast<unsigned long long>(opt->deepen_levels_len),
+                                static_cast<unsigned long long>(f->num));
+                if (target < f->level) target = f->level;
+                item_targets.resize(f->num);
+                for (uint64_t k = 0; k < f->num; ++k) {
+                    const uint64_t from = ordered ? opt->order[k] : k;
+                    const int want = std::max<int>(opt->deepen_levels[from], target);
+                    item_targets[k] = static_cast<uint8_t>(std::min<int>(want, static_cast<int>(a.last) - 1));
+                }
+                if (t.deepen < 0) item_targets.clear();
+            }
+            bool done = false;
+            const bool per_item = !item_targets.empty();
+            if (t.schedule == MF_SCHED_RESPLIT && (target > f->level || per_item) &&
+                target <= static_cast<int>(a.last) - 1 && f->num > 0) {
+                const WorkItem *deep = nullptr;
+                uint64_t deep_num = 0;
+                double deep_ms = 0.0;
+                rc = deepen_frontier(a, core, t, cells, dev, f->num, origins, per_count != nullptr, per_nodes != nullptr,
+                                     static_cast<uint32_t>(target), per_item ? item_targets.data() : nullptr, &deep,
+                                     &deep_num, &deep_ms);
+                if (rc == MF_OK) {
+                    SymArgs a2 = a;
+                    // Every deepened item is at or past the uniform target
+                    // (or its own level, if that was higher).
+                    a2.base_level = static_cast<uint32_t>(std::max(target, f->level));
+                    a2.levels = std::max<uint32_t>(1u, a.last - a2.base_level);
+                    const uint32_t lds2 = sym_lds_bytes(core, a2.levels, t.block, cells);
+                    auto launch2 = [&](const SymArgs &args, bool persistent, uint32_t grid) {
+                        return launch_sym(args, core, persistent, grid, t.block, lds2, t.events, cells, g.stream);
+                    };
+                    const auto t0 = std::chrono::steady_clock::now();
+                    rc = run_solve(a2, ks, kp, lds2, t, deep, deep_num, origins, per_count, per_nodes, nullptr, 0, out,
+                                   true, launch2);
+                    if (rc == MF_OK) {
+                        out->kernel_ms += deep_ms;
+                        out->span_ms += deep_ms;
+                        out->total_ms = ms_since(t0) + deep_ms;
+                    }
+                    done = true;
+                } else if (rc != kDeepenFull) {
+                    return rc;
+                }
+            }
+            if (!done)
+                rc = run_solve(a, ks, kp, lds, t, dev, f->num, origins, per_count, per_nodes, nullptr, 0, out, false,
+                               launch);
         }
     } else {
         LunnonArgs a;
@@ -525,7 +678,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events), lunnon_kernel_ptr(t.core, true, t.events), lds,
                        t, dev_items,
                        f->num, f->origins ? f->origins : f->num, per_count, per_nodes, per_map,
-                       static_cast<uint32_t>(f->maps.size()), out,
+                       static_cast<uint32_t>(f->maps.size()), out, false,
                        launch);
     }
     reordered.release();
@@ -668,7 +821,7 @@ void mf_shutdown(void) {
     g.graph = nullptr;
     g.graph_key.clear();
     for (DevBuf *b : {&g.items, &g.spill_a, &g.spill_b, &g.acc, &g.counters, &g.per_count, &g.per_nodes, &g.per_map,
-                      &g.tables})
+                      &g.tables, &g.deep, &g.targets})
         b->release();
     if (g.host_acc) (void)hipHostFree(g.host_acc);
     if (g.host_counters) (void)hipHostFree(g.host_counters);

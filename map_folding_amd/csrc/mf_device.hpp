@@ -82,7 +82,21 @@ struct SymArgs {
     unsigned long long *acc;    // AccSlot
     unsigned long long *per_count;  // per origin (nullable)
     unsigned long long *per_nodes;  // per origin (nullable)
+    // Deepening pass (sym_deepen_kernel): items below their target level are
+    // expanded by one level; children that reach the target, and input items
+    // already at or past it, go to `final_out`, the others to `spill` with
+    // their target in the mask field (expand items carry no sibling mask).
+    void *final_out;
+    uint32_t *final_count;
+    uint32_t final_cap;
+    uint32_t target;            // first pass: the target of every item without a per-item one
+    const uint8_t *targets;     // first pass: per-item target levels (nullable)
+    uint32_t deep_pass;         // 0: first pass (items are frontier items), 1: later (target in mask)
 };
+
+// ACC_ERROR bits: 1 spill-queue overflow, 2/4 consistency checks, 16 a
+// deepening pass ran out of room (the host then solves without deepening).
+constexpr unsigned long long kErrDeepenFull = 16ull;
 
 struct LunnonArgs {
     const uint8_t *tables;      // concatenated d tables, [ndim][n+1][n+1] bytes each

@@ -69,6 +69,31 @@ __device__ __forceinline__ uint32_t spill_reserve(uint32_t *counter, uint32_t ca
     return __shfl(base, 0, 64);
 }
 
+// The same reservation with per-lane ranges: returns the lane's first slot
+// (lanes' ranges are consecutive in lane order), or kNoSlot in every lane if
+// the wave's total does not fit (nothing reserved).
+__device__ __forceinline__ uint32_t lane_reserve(uint32_t *counter, uint32_t cap, uint32_t need, uint32_t lane) {
+    uint32_t incl = need;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= static_cast<uint32_t>(off)) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (total == 0) return 0;
+    uint32_t base = kNoSlot;
+    if (lane == 0) {
+        const uint32_t old = atomicAdd(counter, total);
+        if (old <= cap && total <= cap - old) {
+            base = old;
+        } else {
+            atomicSub(counter, total);
+        }
+    }
+    base = __shfl(base, 0, 64);
+    return base == kNoSlot ? kNoSlot : base + (incl - need);
+}
+
 // Copy 64 state bytes between an item and a lane's lane-interleaved LDS rows
 // (dword w of the lane at lane_base + w*256).
 __device__ __forceinline__ void state_to_lds(uint8_t *lds, uint32_t lane_base, const uint8_t *src) {

@@ -687,6 +687,96 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     flush_wave(a, lane, tot);
 }
 
+// ------------------------------------------------------------- deepening
+// One level of the frontier deepening that runs before the persistent rounds
+// (u64 geometries): one lane per item.  An expand item below its target level
+// is expanded -- one get_good, counted as a node exactly as the DFS counts it
+// -- and each child (ascending insertion point, so siblings stay together)
+// goes to the final array if it reaches the target, else to the next pass.
+// Items at or past their target, and non-expand items, are copied to the
+// final array unchanged (a non-expand item below the target cannot occur:
+// the frontiers hold expand items only; it raises kErrDeepenFull).  Deeper
+// items leave a shorter stack per lane in the persistent kernel, and more
+// workgroups fit the LDS of a CU.
+template <typename G, bool EV>
+__global__ __launch_bounds__(256) void sym_deepen_kernel(SymArgs a) {
+    using Mask = typename G::Mask;
+    using Item = typename G::Item;
+    extern __shared__ __align__(16) uint8_t lds[];
+    stage_tables<G>(a, lds);
+    const uint32_t lane = threadIdx.x & 63u;
+    const Lane ln = make_lane<G>(1, 0);
+    const auto *items = static_cast<const Item *>(a.items);
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool have = idx < a.num_items;
+    uint32_t level = 0, target = 0, origin = 0, mult = 0;
+    bool expand = false;
+    Mask good = 0;
+    uint32_t nw = 0, ns = 0;
+    if (have) {
+        const Item *it = items + idx;
+        level = it->level;
+        origin = it->origin;
+        mult = it->mult;
+        const Mask m = G::item_mask(it);
+        if (a.deep_pass == 0) {
+            target = a.targets ? a.targets[idx] : a.target;
+            expand = m == 0 && it->map == 0 && level < target;
+            if ((m != 0 || it->map != 0) && level < target) atomicOr(a.acc + ACC_ERROR, kErrDeepenFull);
+        } else {
+            target = static_cast<uint32_t>(m);
+            expand = level < target;
+        }
+        if (expand) {
+            row_load<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, it->state);
+            good = good_mask<G, EV>(lds, ln.cl, level, nw, ns);
+        }
+    }
+    // Slots: the children of an expanded item, or the item itself.
+    const uint32_t cnt = expand ? mask_popcount(good) : 0u;
+    const bool to_final = !expand || level + 1 >= target;
+    const uint32_t nf = have ? (expand ? (to_final ? cnt : 0u) : 1u) : 0u;
+    const uint32_t nc = (have && expand && !to_final) ? cnt : 0u;
+    const uint32_t fslot = lane_reserve(a.final_count, a.final_cap, nf, lane);
+    const uint32_t cslot = lane_reserve(a.spill_count, a.spill_cap, nc, lane);
+    if (fslot == kNoSlot || cslot == kNoSlot) {
+        if (lane == 0) atomicOr(a.acc + ACC_ERROR, kErrDeepenFull);
+        return;
+    }
+    if (have && !expand) {
+        static_cast<Item *>(a.final_out)[fslot] = items[idx];
+        if (a.deep_pass != 0) G::set_item_mask(static_cast<Item *>(a.final_out) + fslot, 0);
+    }
+    unsigned long long nodes = 0, walks = 0, steps = 0;
+    if (expand) {
+        nodes = 1;
+        if (EV) {
+            walks = nw;
+            steps = ns;
+        }
+        if (a.per_nodes != nullptr) atomicAdd(a.per_nodes + origin, 1ull);
+        Item *dst = to_final ? static_cast<Item *>(a.final_out) + fslot : static_cast<Item *>(a.spill) + cslot;
+        const Mask carry = to_final ? static_cast<Mask>(0) : static_cast<Mask>(target);
+        for (Mask gm = good; gm != 0; gm &= gm - 1, ++dst) {
+            const uint32_t j = mask_ctz(gm);
+            place<G>(lds, ln, j, level);
+            row_store<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, dst->state);
+            unplace<G>(lds, ln, j, level);
+            G::set_item_mask(dst, carry);
+            dst->origin = origin;
+            dst->mult = static_cast<uint16_t>(mult);
+            dst->level = static_cast<uint8_t>(level + 1);
+            dst->map = 0;
+        }
+    }
+    const unsigned long long n = wave_sum(nodes), wk = wave_sum(walks), st = wave_sum(steps);
+    if (lane == 0) {
+        if (n) atomicAdd(a.acc + ACC_NODES, n);
+        if (wk) atomicAdd(a.acc + ACC_WALKS, wk);
+        if (st) atomicAdd(a.acc + ACC_STEPS, st);
+    }
+}
+
 // Host-visible entry points (C++ linkage inside the library).
 namespace {
 template <typename G>
@@ -736,6 +826,17 @@ uint32_t lds_of(uint32_t levels, uint32_t block) {
     return G::kTableBytes + (block / 64u) * (G::kCycleBytes + levels * G::Stack::kSlotBytes);
 }
 }  // namespace
+
+// Deepening pass for the u64 geometries (nullptr for 9x9 and the wide stack).
+const void *sym_deepen_ptr(int core, bool events, uint32_t cells) {
+    switch (geometry(core, cells)) {
+        case Geo::k49: return events ? reinterpret_cast<const void *>(&sym_deepen_kernel<G49, true>)
+                                     : reinterpret_cast<const void *>(&sym_deepen_kernel<G49, false>);
+        case Geo::k64: return events ? reinterpret_cast<const void *>(&sym_deepen_kernel<G64, true>)
+                                     : reinterpret_cast<const void *>(&sym_deepen_kernel<G64, false>);
+        default: return nullptr;
+    }
+}
 
 uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells) {
     switch (geometry(core, cells)) {

@@ -329,3 +329,53 @@ def test_learned_split_7x7_window(gpu, golden):
         assert int(pn.sum()) + pieces.host_nodes == ref.nodes, sched
     f.close()
     model.close()
+
+
+@pytest.mark.parametrize("n,e,start,num,targets", [(8, 36, 120000000, 2000, (40, 47, 50)), (7, 24, 200000, 1000, (28, 30)),
+                                                   (8, 28, 1000, 200, (36, 44))])
+def test_deepened_frontier_keeps_every_count(gpu, n, e, start, num, targets):
+    """The re-split schedule on a frontier deepened on the device to a target
+    level (one counted get_good per expanded node) gives the per-case counts,
+    per-case nodes and exact event counters of the undeepened solve, at any
+    target, budget and round mode."""
+    _say(f"deepening: {n}x{n} level {e} [{start},+{num}) targets {targets}")
+    cyc, mul = gpu.sym_cases(n, e, start, num)
+    f = gpu.SymFrontier(n, e, cyc, mul)
+    ref, pc0, pn0 = f.solve(_opts(gpu, "resplit", deepen=-1), per_item=True)
+    for target in targets:
+        for sched in ("resplit", "resplit_tight", "resplit_tight_nograph"):
+            r, pc, pn = f.solve(_opts(gpu, sched, deepen=target), per_item=True)
+            assert (r.count, r.nodes, r.walks, r.steps, r.leaves) == (ref.count, ref.nodes, ref.walks, ref.steps,
+                                                                      ref.leaves), (target, sched)
+            assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0), (target, sched)
+    f.close()
+
+
+def test_deepening_out_of_room_falls_back(gpu):
+    """A target whose deepened frontier would not fit the device buffers is
+    abandoned and the frontier solved as it is: the counts do not change."""
+    _say("deepening fallback: 8x8 level 36 [0,+300) to level 61")
+    cyc, mul = gpu.sym_cases(8, 36, 0, 300)
+    ref, pc0, pn0 = gpu.sym_solve(8, 36, cyc, mul, _opts(gpu, "resplit", deepen=-1), per_item=True)
+    r, pc, pn = gpu.sym_solve(8, 36, cyc, mul, _opts(gpu, "resplit", deepen=61), per_item=True)
+    assert (r.count, r.nodes, r.walks, r.steps) == (ref.count, ref.nodes, ref.walks, ref.steps)
+    assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0)
+
+
+def test_per_item_deepening_targets_keep_counts(gpu):
+    """Per-item deepening targets (as the cost model picks them), with and
+    without an item order, give the counts of the undeepened solve."""
+    _say("per-item deepening targets: 8x8 level 36 [120000000,+1500)")
+    cyc, mul = gpu.sym_cases(8, 36, 120000000, 1500)
+    f = gpu.SymFrontier(8, 36, cyc, mul)
+    ref, pc0, pn0 = f.solve(_opts(gpu, "resplit", deepen=-1), per_item=True)
+    rng = np.random.default_rng(3)
+    lv = rng.integers(36, 52, size=1500).astype(np.uint8)
+    for order in (None, rng.permutation(1500)):
+        for sched in ("resplit", "resplit_tight"):
+            r, pc, pn = f.solve(_opts(gpu, sched, deepen=44, deepen_levels=lv, order=order), per_item=True)
+            assert (r.count, r.nodes, r.walks, r.steps) == (ref.count, ref.nodes, ref.walks, ref.steps), sched
+            assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0), sched
+    with pytest.raises(gpu.MFError):
+        f.solve(_opts(gpu, "resplit", deepen_levels=lv[:10]))
+    f.close()
