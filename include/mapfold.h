@@ -266,6 +266,8 @@ int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt,
 /* Solve a resident Lunnon frontier and return each map's exact total. */
 int mf_frontier_solve_maps(mf_frontier *f, const mf_solve_options *opt,
                            uint64_t *per_map_count, mf_result *out);
+/* Work items resident in the frontier (cases, states or pieces). */
+uint64_t mf_frontier_items(const mf_frontier *f);
 void mf_frontier_destroy(mf_frontier *f);
 
 /* ---- sym_kernel.c-compatible batch ABI (mfah wire format) --------------- */

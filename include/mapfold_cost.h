@@ -28,6 +28,8 @@
 
 #include <stdint.h>
 
+#include "mapfold.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -82,6 +84,14 @@ int mf_sym_estimate_work(int32_t n, int32_t level, const uint8_t *cycles, int64_
 /* Predictions: BF16 MFMA inference, and the FP32 training forward. */
 int mf_cost_predict(mf_cost_model *m, const float *x, int64_t count, float *pred);
 int mf_cost_predict_fp32(mf_cost_model *m, const float *x, int64_t count, float *pred);
+/* The same prediction for every item of a resident sym frontier (n <= 8)
+ * without a host round trip: the feature rows (the values of
+ * mf_cost_features_sym) are computed on the device from the resident items,
+ * then the BF16 inference runs on them.  *rows = the frontier's items;
+ * pred needs room for them (max_rows). */
+int mf_cost_predict_frontier(mf_cost_model *m, mf_frontier *f, float *pred, int64_t max_rows, int64_t *rows);
+/* The device-computed feature rows themselves (80 floats per item). */
+int mf_cost_features_frontier(mf_frontier *f, float *x, int64_t max_rows, int64_t *rows);
 
 #ifdef __cplusplus
 }

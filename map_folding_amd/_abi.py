@@ -137,6 +137,9 @@ def lib() -> C.CDLL:
         "mf_sym_estimate_work": (C.c_int, [i32, i32, P(u8), i64, i32, P(u64)]),
         "mf_cost_predict": (C.c_int, [C.c_void_p, P(f32), i64, P(f32)]),
         "mf_cost_predict_fp32": (C.c_int, [C.c_void_p, P(f32), i64, P(f32)]),
+        "mf_cost_predict_frontier": (C.c_int, [C.c_void_p, C.c_void_p, P(f32), i64, P(i64)]),
+        "mf_cost_features_frontier": (C.c_int, [C.c_void_p, P(f32), i64, P(i64)]),
+        "mf_frontier_items": (u64, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

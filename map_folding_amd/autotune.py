@@ -2,7 +2,7 @@
 
 Every knob here only changes how work is cut and scheduled -- the Lunnon
 partition depth per map, the re-split budget and tail, the resident waves per
-CU, the learned split threshold -- never a count, so the tuner picks the
+CU, the sym deepening target, the learned split threshold -- never a count, so the tuner picks the
 fastest configuration by measuring real solves on the GPU (best of `repeats`
 kernel times) and checks every trial's result against the first one.
 
@@ -21,7 +21,7 @@ import time
 
 from . import LunnonBatchFrontier, SolveOptions, lunnon_frontier_size
 
-__all__ = ["tune_lunnon_batch", "tune_resplit", "depths_for_target", "save", "load"]
+__all__ = ["tune_lunnon_batch", "tune_resplit", "tune_sym", "depths_for_target", "save", "load"]
 
 
 def _best(fn, repeats):
@@ -131,6 +131,67 @@ def tune_resplit(frontier, budgets=(32, 64, 128), tail_mult=(1, 4), waves=(20, 3
                     best = {"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "kernel_ms": ms}
     best["trials"] = trials
     return best
+
+
+def tune_sym(frontier, pred=None, level=None, deepens=(45, 46, 47, 48), thetas=(19, 21, 23), caps=(2, 4),
+             budgets=(64, 128), waves=32, repeats=1, log=None):
+    """The symmetry DFS's partition depth and split threshold on a resident
+    case frontier: the device deepening target (every case expanded to that
+    level before the rounds: fewer stack levels per lane, more resident
+    waves), then -- given the cost model's predictions `pred` (log2 nodes per
+    case) -- the learned threshold theta and the cap on extra levels for the
+    cases predicted large (cost.learned_levels), then the re-split budget.
+    Each stage keeps the best of the previous one.  `level` is the cases'
+    level (deepening targets at or below it are skipped).
+
+    Returns {"deepen", "theta", "cap", "budget", "kernel_ms", "trials"};
+    theta/cap are None when the learned levels did not beat the uniform target."""
+    from . import cost
+    say = log or (lambda m: None)
+    trials = []
+    ref = None
+
+    def trial(name, deepen, levels, budget):
+        nonlocal ref
+        opt = SolveOptions(count_events=False, budget=budget, tail=budget, waves_per_cu=waves, deepen=deepen,
+                           deepen_levels=levels)
+
+        def once():
+            r, _, _ = frontier.solve(opt)
+            return r.kernel_ms, (r.count, r.nodes)
+        ms, res = _best(once, repeats)
+        if ref is None:
+            ref = res
+        elif res != ref:
+            raise RuntimeError("a tuning trial changed the count")
+        trials.append({"trial": name, "deepen": deepen, "budget": budget, "kernel_ms": ms})
+        say(f"  {name}: {ms:.1f} ms")
+        return ms
+
+    budget = budgets[0]
+    best = (trial("no deepening", -1, None, budget), -1)
+    for d in deepens:
+        if level is not None and d <= level:
+            continue
+        ms = trial(f"deepen {d}", d, None, budget)
+        if ms < best[0]:
+            best = (ms, d)
+    deepen = best[1]
+    best_ms = best[0]
+    theta = cap = None
+    if pred is not None and deepen > 0:
+        for th in thetas:
+            for cp in caps:
+                ms = trial(f"deepen {deepen} + learned theta {th} cap {cp}", deepen,
+                           cost.learned_levels(pred, deepen, th, cp), budget)
+                if ms < best_ms:
+                    best_ms, theta, cap = ms, th, cp
+    levels = None if theta is None else cost.learned_levels(pred, deepen, theta, cap)
+    for bu in budgets[1:]:
+        ms = trial(f"budget {bu}", deepen, levels, bu)
+        if ms < best_ms:
+            best_ms, budget = ms, bu
+    return {"deepen": deepen, "theta": theta, "cap": cap, "budget": budget, "kernel_ms": best_ms, "trials": trials}
 
 
 def save(cfg: dict, path: str) -> None:

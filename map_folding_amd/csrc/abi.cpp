@@ -32,6 +32,7 @@ uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
+hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s);
 }  // namespace mf
 
 using namespace mf;
@@ -760,6 +761,20 @@ bool lib_current(LibContext *c) {
     return true;
 }
 int lib_error(int status, const char *msg) { return fail(status, "%s", msg); }
+// Cost-model feature rows (80 floats each, device memory) of a resident sym
+// frontier of up to 8x8, on the library stream.
+int frontier_sym_features(mf_frontier *f, float *dev_out, uint64_t *rows) {
+    if (!f || f->kind != 1) return fail(MF_ERR_ARG, "features on the device take a sym frontier of n <= 8");
+    SymArgs a;
+    uint32_t levels = 0;
+    sym_args(*f, a, &levels);
+    a.items = f->items.ptr;
+    a.num_items = static_cast<uint32_t>(f->num);
+    *rows = f->num;
+    if (f->num == 0) return MF_OK;
+    HIP_TRY(launch_sym_features(a, dev_out, g.stream));
+    return MF_OK;
+}
 int lib_context(hipStream_t *stream, int *cus) {
     const int rc = require_init();
     if (rc != MF_OK) return rc;
@@ -1305,6 +1320,8 @@ int mf_frontier_solve(mf_frontier *f, const mf_solve_options *opt, uint64_t *per
     if (!f || !out) return fail(MF_ERR_ARG, "frontier/out is NULL");
     return solve_frontier(f, opt, per_count, per_nodes, nullptr, out);
 }
+
+uint64_t mf_frontier_items(const mf_frontier *f) { return f ? f->num : 0; }
 
 void mf_frontier_destroy(mf_frontier *f) {
     if (!f) return;

@@ -22,6 +22,7 @@
 namespace mf {
 int lib_error(int status, const char *msg);
 int lib_context(hipStream_t *stream, int *cus);
+int frontier_sym_features(mf_frontier *f, float *dev_out, uint64_t *rows);
 }  // namespace mf
 
 using namespace mf;
@@ -555,6 +556,55 @@ int mf_cost_predict(mf_cost_model *m, const float *x, int64_t count, float *pred
                              m->small.f(), m->pred.f(), cus, s));
     COST_TRY(hipMemcpyAsync(pred, m->pred.p, N * 4, hipMemcpyDeviceToHost, s));
     COST_TRY(hipStreamSynchronize(s));
+    return MF_OK;
+}
+
+int mf_cost_predict_frontier(mf_cost_model *m, mf_frontier *f, float *pred, int64_t max_rows, int64_t *rows) {
+    if (!m || !f || !rows) return lib_error(MF_ERR_ARG, "bad arguments");
+    hipStream_t s;
+    int cus, rc;
+    if ((rc = context(&s, &cus)) != MF_OK) return rc;
+    uint64_t n = mf_frontier_items(f);
+    *rows = static_cast<int64_t>(n);
+    if (n == 0) return MF_OK;
+    if (static_cast<int64_t>(n) > max_rows || !pred)
+        return lib_error(MF_ERR_CAPACITY, "pred has room for fewer rows than the frontier has items");
+    if (!m->frag_valid) {
+        COST_TRY(cost_pack_bf16(m->params.f(), static_cast<uint16_t *>(m->frag.p), m->small.f(), s));
+        m->frag_valid = true;
+    }
+    COST_TRY(m->data_x.ensure(n * kFeatures * 4));
+    COST_TRY(m->pred.ensure(n * 4));
+    uint64_t got = 0;
+    if ((rc = frontier_sym_features(f, m->data_x.f(), &got)) != MF_OK) return rc;
+    COST_TRY(cost_infer_bf16(static_cast<int>(n), m->data_x.f(), static_cast<const uint16_t *>(m->frag.p),
+                             m->small.f(), m->pred.f(), cus, s));
+    COST_TRY(hipMemcpyAsync(pred, m->pred.p, n * 4, hipMemcpyDeviceToHost, s));
+    COST_TRY(hipStreamSynchronize(s));
+    return MF_OK;
+}
+
+int mf_cost_features_frontier(mf_frontier *f, float *x, int64_t max_rows, int64_t *rows) {
+    if (!f || !rows) return lib_error(MF_ERR_ARG, "bad arguments");
+    hipStream_t s;
+    int cus, rc;
+    if ((rc = context(&s, &cus)) != MF_OK) return rc;
+    const uint64_t n = mf_frontier_items(f);
+    *rows = static_cast<int64_t>(n);
+    if (n == 0) return MF_OK;
+    if (static_cast<int64_t>(n) > max_rows || !x)
+        return lib_error(MF_ERR_CAPACITY, "x has room for fewer rows than the frontier has items");
+    Buf dev;
+    COST_TRY(dev.ensure(n * kFeatures * 4));
+    uint64_t got = 0;
+    if ((rc = frontier_sym_features(f, dev.f(), &got)) != MF_OK) {
+        dev.release();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(x, dev.p, n * kFeatures * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev.release();
+    COST_TRY(e);
     return MF_OK;
 }
 

@@ -777,6 +777,130 @@ __global__ __launch_bounds__(256) void sym_deepen_kernel(SymArgs a) {
     }
 }
 
+// ------------------------------------------------------------- cost features
+// The learned cost model's feature row of every case of a resident frontier,
+// computed on the device from the case's own cycle -- the same 80 values as
+// the host's sym_row (cost.cpp), which the model is trained on:
+//   [0, 64)   placed cell c: (rank of c along the cycle from cell 0 + 1) / 64;
+//             unplaced cell c < n*n: popcount of the intersection of the
+//             page walks from its placed partners (cells below the level) /
+//             level; 0 past n*n;
+//   64..71    level / 64, level / n*n, children / 16, n*n / 64, 1 (sym),
+//             constrained pages / 4, levels left / 64, n / 8;
+//   [72, 80)  log2(1 + descendants at level + k) / 24, k = 1..8: a DFS to
+//             depth 8 (estimate_work of sym_fold.cpp:235-250 at each depth).
+// One lane per case.  LDS: the partner tables, each lane's cycle row, and per
+// lane the DFS's remaining-children masks and current children (8 levels).
+constexpr uint32_t kFeatureDepth = 8;
+constexpr uint32_t kFeatureStackLevels = 36;   // 36 x 128 B per wave: 8 u64 masks + 8 child bytes per lane
+
+template <typename G>
+__global__ __launch_bounds__(256) void sym_features_kernel(SymArgs a, float *out) {
+    using Mask = typename G::Mask;
+    using Item = typename G::Item;
+    extern __shared__ __align__(16) uint8_t lds[];
+    stage_tables<G>(a, lds);
+    const uint32_t lane = threadIdx.x & 63u;
+    const Lane ln = make_lane<G>(kFeatureStackLevels, 0);
+    const uint32_t sbase = ln.sl - lane * G::Stack::kLaneBytes;   // this wave's stack region
+    unsigned long long *rem = reinterpret_cast<unsigned long long *>(lds + sbase) + lane;   // [k * 64]
+    uint8_t *cur = lds + sbase + kFeatureDepth * 512u + lane;                            // [k * 64]
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.num_items) return;
+    const Item *it = static_cast<const Item *>(a.items) + idx;
+    const uint32_t level = it->level;
+    const uint32_t n2 = a.last + 1;
+    row_load<G::kStateBytes, G::kRowCells, G::kRowStride>(lds, ln.cl, it->state);
+    float *f = out + static_cast<size_t>(idx) * 80u;
+    // Placed cells: rank along the cycle from cell 0.
+    uint32_t c = 0;
+    for (uint32_t r = 0; r < level; ++r) {
+        f[c] = static_cast<float>(r + 1) / 64.0f;
+        c = cyc_ld<G>(lds, ln.cl, c);
+        if (c == 0) break;
+    }
+    // Unplaced cells: their options against the current stack.
+    uint32_t dummy = 0;
+    for (uint32_t cell = level; cell < G::kCells; ++cell) {
+        float v = 0.0f;
+        if (cell < n2) {
+            Mask opts = G::below(level);
+            for (uint32_t p = 0; p < 4 && opts != 0; ++p) {
+                const uint32_t anchor = lds[G::kPageOff + G::kCells * p + cell];
+                if (anchor < level) opts &= page_walk<G>(lds, ln.cl, level, page_row<G>(p), anchor, dummy);
+            }
+            v = static_cast<float>(mask_popcount(opts)) / static_cast<float>(level);
+        }
+        f[cell] = v;
+    }
+    // Lookahead profile: descendants at depths 1..8 (one iterative DFS).
+    unsigned long long prof[kFeatureDepth] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t k = 0, i = level;
+    uint32_t nw = 0, ns = 0;
+    for (;;) {
+        bool descend = false;
+        if (i < n2) {
+            const Mask g = good_mask<G, false>(lds, ln.cl, i, nw, ns);
+            const uint32_t cnt = mask_popcount(g);
+#pragma unroll
+            for (uint32_t q = 0; q < kFeatureDepth; ++q) prof[q] += (q == k) ? cnt : 0u;
+            if (k + 1 < kFeatureDepth && i + 1 < n2 && g != 0) {
+                const uint32_t j = mask_ctz(g);
+                rem[k * 64u] = static_cast<unsigned long long>(g & (g - 1));
+                cur[k * 64u] = static_cast<uint8_t>(j);
+                place<G>(lds, ln, j, i);
+                ++k;
+                ++i;
+                descend = true;
+            }
+        }
+        if (descend) continue;
+        bool resumed = false;
+        while (k > 0) {
+            --k;
+            --i;
+            unplace<G>(lds, ln, cur[k * 64u], i);
+            unsigned long long r = rem[k * 64u];
+            if (r != 0) {
+                const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(r));
+                rem[k * 64u] = r & (r - 1);
+                cur[k * 64u] = static_cast<uint8_t>(j);
+                place<G>(lds, ln, j, i);
+                ++k;
+                ++i;
+                resumed = true;
+                break;
+            }
+        }
+        if (!resumed) break;
+    }
+    uint32_t pages = 0;
+    for (uint32_t p = 0; p < 4; ++p)
+        pages += (level < n2 && lds[G::kPageOff + G::kCells * p + level] < level) ? 1u : 0u;
+    f[64] = static_cast<float>(level) / 64.0f;
+    f[65] = static_cast<float>(level) / static_cast<float>(n2);
+    f[66] = static_cast<float>(prof[0]) / 16.0f;
+    f[67] = static_cast<float>(n2) / 64.0f;
+    f[68] = 1.0f;
+    f[69] = static_cast<float>(pages) / 4.0f;
+    f[70] = static_cast<float>(n2 - level) / 64.0f;
+    const uint32_t n = static_cast<uint32_t>(sqrtf(static_cast<float>(n2)) + 0.5f);   // side of the square
+    f[71] = static_cast<float>(n) / 8.0f;
+#pragma unroll
+    for (uint32_t q = 0; q < kFeatureDepth; ++q)
+        f[72 + q] = static_cast<float>(log2(1.0 + static_cast<double>(prof[q]))) / 24.0f;
+}
+
+hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s) {
+    // Rows of 64 cells for every square up to 8x8.
+    const uint32_t lds = G64::kTableBytes + 4u * (G64::kCycleBytes + kFeatureStackLevels * G64::Stack::kSlotBytes);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sym_features_kernel<G64>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sym_features_kernel<G64>, dim3((a.num_items + 255u) / 256u), dim3(256), lds, s, a, out);
+    return hipGetLastError();
+}
+
 // Host-visible entry points (C++ linkage inside the library).
 namespace {
 template <typename G>

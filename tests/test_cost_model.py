@@ -194,3 +194,32 @@ def test_learned_model_gains_over_no_model(gpu):
     lookahead = cost.wave_waste(nodes[cost.lookahead_order(proxy, mul)])
     assert learned < 0.6 * stream and learned < lookahead, (stream, learned, lookahead)
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,e,start,num", [(8, 36, 120000000, 3000), (7, 24, 200000, 2000), (6, 16, 0, 1225)])
+def test_device_features_equal_host_features(gpu, n, e, start, num):
+    """The feature rows computed on the device from a resident frontier equal
+    the host's (cost.cpp sym_row) -- exactly for the ranks, options and
+    scalars, to float rounding for the log2 lookahead profile -- and the
+    on-device prediction equals the prediction of the host rows."""
+    import os
+    from map_folding_amd import cost
+    if n == 6:
+        cyc, mul, _ = gpu.sym_seeds(6)
+    else:
+        cyc, mul = gpu.sym_cases(n, e, start, num)
+    f = gpu.SymFrontier(n, e, cyc, mul)
+    xd = cost.features_frontier(f)
+    xh = cost.features_sym(n, e, cyc)
+    assert xd.shape == xh.shape
+    assert np.array_equal(xd[:, :72], xh[:, :72])
+    assert np.allclose(xd[:, 72:], xh[:, 72:], rtol=0, atol=1e-6)
+    m = cost.CostModel(seed=0)
+    m.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "map_folding_amd",
+                        "cost_default.npy"))
+    pd = m.predict_frontier(f)
+    ph = m.predict(xh)
+    assert np.allclose(pd, ph, rtol=0, atol=1e-3)
+    f.close()
+    m.close()
