@@ -225,6 +225,15 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
             HIP_TRY(launch(args, false, grid));
             HIP_TRY(hipEventRecord(g.ev1, g.stream));
             rounds = 1;
+            // One launch bounded by its longest item: it can run for minutes,
+            // so wait by polling its end event instead of a spinning
+            // synchronize that would hold a host core the whole time.
+            for (;;) {
+                const hipError_t q = hipEventQuery(g.ev1);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) return fail(MF_ERR_HIP, "hipEventQuery failed: %s", hipGetErrorString(q));
+                std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            }
         } else {
             if ((rc = set_lds_limit(kpersist, lds_bytes)) != MF_OK) return rc;
             uint32_t grid = 0;
