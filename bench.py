@@ -22,8 +22,8 @@ line:
   * plain         the plain static-frontier kernel (one lane per case, one
                   launch) on the whole chunk, timed while the CPU baselines run
                   on the host (it is bounded by its largest case, minutes);
-  * cpu_baseline  the reference's own CPU solver built from its sources
-                  (oracle/_ref), on a bounded window of the same stream;
+  * cpu_baseline  the reference's own CPU command built from its sources
+                  (oracle/_ref) on the same chunk, on the host's cores;
   * configs       5x5 Lunnon depth 15, 6x6 Lunnon depth 18, the 64 seeded
                   rectangles in one queue, and a 20,000-case 7x7 window, each
                   with its time, rate, roofline fraction, bit-exactness and CPU
@@ -51,10 +51,10 @@ SYM_WORKLOADS = {
 }
 # Deepening targets the tuner tries per square (the cases' level must be below).
 DEEPENS = {8: (45, 46, 47, 48), 7: (28, 30, 32, 34), 9: ()}
-# CPU-baseline windows of the reference's sym_fold / sym_fold9 (binary, start, count): 10-30 s of host work.
-# The default run times the plain static kernel (several minutes) while these
-# run on the host, so they are sized to fill that time: ~170 s in all.
-REF_WINDOWS = {"sym8": ("sym_fold", 0, 40000), "sym7": ("sym_fold", 200000, 20000), "sym9": ("sym_fold9", 0, 2000)}
+# The reference CPU solver of each square for the CPU baselines, run on the
+# very window the GPU times (the reference's own command for it).  The default
+# run times the plain static kernel (minutes) while the host runs them.
+REF_EXE = {7: "sym_fold", 8: "sym_fold", 9: "sym_fold9"}
 # Oracle-port samples (cases): the first cases of the timed window.
 PORT_CASES = {"sym8": 6000, "sym7": 6000, "sym9": 500}
 
@@ -130,21 +130,32 @@ def _run_ref(exe_name, argv, env_extra=None):
 # answers for the same windows are computed beforehand.
 
 
-def sym_ref_job(workload, n, level, gpu_answer, gpu_nodes, threads):
-    exe_name, start, count = REF_WINDOWS[workload]
+def sym_ref_job(exe_name, n, level, start, count, gpu_answer, gpu_nodes, threads):
+    """The reference's own command on the window [start, start+count):
+    `MF_STREAM_CASES=1 MF_EXPANDED_N=level sym_fold n threads count start`.
+    Its producer enumerates every case before `start` on one thread first, so
+    the same command with one case (setup + skip) is timed as well and the
+    rate is taken over the difference."""
+    env = {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": str(level)}
 
     def job():
-        got = _run_ref(exe_name, [n, threads, count, start], {"MF_STREAM_CASES": "1", "MF_EXPANDED_N": str(level)})
+        got = _run_ref(exe_name, [n, threads, 1, start], env)
         if got is None:
             return None
-        out, cs = got
+        skip_s = got[1]
+        out, cs = _run_ref(exe_name, [n, threads, count, start], env)
         answer = int(out.split()[0])
         if answer != gpu_answer:
             raise SystemExit(f"reference {exe_name} answered {answer}, the GPU {gpu_answer}")
-        return {"value": gpu_nodes / cs, "unit": "nodes/s", "cores": threads, "kind": "reference",
+        solve_s = max(cs - skip_s, 1e-3)
+        return {"value": gpu_nodes / solve_s, "unit": "nodes/s", "cores": threads, "kind": "reference",
+                "wall_s": cs, "setup_and_skip_s": skip_s, "nodes_per_s_incl_skip": gpu_nodes / cs,
                 "sample": f"oracle/_ref/{exe_name} (reference source built here) {n} {threads} {count} {start} "
-                          f"with MF_STREAM_CASES=1 MF_EXPANDED_N={level}: {count} cases ({gpu_nodes} nodes, answer "
-                          f"{answer} = the GPU's), {cs:.1f} s wall including its seed and stream setup"}
+                          f"with MF_STREAM_CASES=1 MF_EXPANDED_N={level}: the same {count} cases as the GPU "
+                          f"({gpu_nodes} nodes, answer {answer} = the GPU's), {cs:.1f} s wall, of which "
+                          f"{skip_s:.1f} s is its seed setup and one-thread enumeration of the {start} cases "
+                          f"before the window (timed alone with max_cases 1); rate over the remaining "
+                          f"{solve_s:.1f} s"}
     return job
 
 
@@ -160,9 +171,21 @@ def sym_port_job(n, level, cyc, mul, threads):
             src = "sym_fold9.cpp recurse_count (u128 masks)"
         cs = time.perf_counter() - t1
         return {"value": st["nodes"] / cs, "unit": "nodes/s", "cores": threads, "kind": "port",
+                "cases": int(cyc.shape[0]), "nodes": int(st["nodes"]), "seconds": cs, "source": src,
                 "sample": f"first {cyc.shape[0]} cases of the window ({st['nodes']} nodes), oracle C restatement of "
                           f"{src} on {threads} threads, {cs:.1f} s"}
     return job
+
+
+def extend_port(rec, more):
+    """Fold the next block of the window into an oracle-port sample."""
+    rec["cases"] += more["cases"]
+    rec["nodes"] += more["nodes"]
+    rec["seconds"] += more["seconds"]
+    rec["value"] = rec["nodes"] / rec["seconds"]
+    rec["sample"] = (f"first {rec['cases']} cases of the window ({rec['nodes']} nodes), oracle C restatement of "
+                     f"{rec['source']} on {rec['cores']} threads, {rec['seconds']:.1f} s")
+    return rec
 
 
 # ----------------------------------------------------------------------------- sym workloads
@@ -301,7 +324,7 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
            "schedule": "persistent re-split, device deepening" + ("" if levels is None else " + learned per-case depth"),
            "tuned": {k: cfg[k] for k in ("deepen", "theta", "cap", "budget", "tune_s")},
            "learned": learned, **comp, "end_to_end_ms": e2e}
-    return rec, {"n": n, "level": level, "cyc": cyc, "mul": mul, "front": front, "total": total,
+    return rec, {"n": n, "level": level, "start": start, "cyc": cyc, "mul": mul, "front": front, "total": total,
                  "nodes": res.nodes, "walks": res.walks, "steps": res.steps, "case_nodes": case_nodes,
                  "n_inputs": cyc.shape[0], "item_bytes": 80 if n * n <= 64 else 128}
 
@@ -451,26 +474,36 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
         jobs.append(port_job)
     else:
         def ref_job():
-            from oracle import oracle
-            small, inserts, total_s = [], 0, 0.0
-            for p, q in shapes:
-                if p * q > 27 or (p, q) in small:
-                    continue
-                tot, st = oracle.lunnon_count([p, q])
-                got = _run_ref("mf", [p, q])
-                if got is None:
-                    return None
-                out, cs = got
-                if int(out.split()[0]) != tot:
-                    raise SystemExit(f"reference mf {p} {q} disagreed with the oracle")
-                small.append((p, q))
-                inserts += st["inserts"]
-                total_s += cs
-                if total_s > 20.0:
-                    break
-            return {"value": inserts / total_s, "unit": "inserts/s", "cores": 1, "kind": "reference",
-                    "sample": f"oracle/_ref/mf p q (reference source built here) on the draw's maps {small}, "
-                              f"answers = the oracle's, {inserts} inserts (the oracle's exact count), {total_s:.1f} s"}
+            # `mf p q` for every map of the draw, `threads` processes at a
+            # time, largest first; the insert counts of each full tree are the
+            # oracle's exact event counts (golden table).
+            from concurrent.futures import ThreadPoolExecutor
+            if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "mf")):
+                return None
+            with open(os.path.join(ROOT, "tests", "golden", "rect_inserts.json")) as f:
+                ins = json.load(f)["inserts"]
+            order = sorted(range(len(shapes)), key=lambda i: -ins["%dx%d" % shapes[i]])
+            t1 = time.perf_counter()
+
+            def one(i):
+                p, q = shapes[i]
+                out, cs = _run_ref("mf", [p, q])
+                if int(out.split()[0]) != want_maps[i]:
+                    raise SystemExit(f"reference mf {p} {q} disagreed with the golden table")
+                if cs > 5.0:
+                    log(f"cpu baseline rect64: mf {p} {q} done in {cs:.1f} s "
+                        f"({time.perf_counter() - t1:.0f} s into the batch)")
+                return cs
+            with ThreadPoolExecutor(max_workers=threads) as pool:
+                secs = list(pool.map(one, order))
+            wall = time.perf_counter() - t1
+            inserts = sum(ins["%dx%d" % s] for s in shapes)
+            return {"value": inserts / wall, "unit": "inserts/s", "cores": threads, "kind": "reference",
+                    "sample": f"oracle/_ref/mf p q (reference source built here) for all {len(shapes)} maps of the "
+                              f"draw, {threads} processes at a time, largest first; every answer = the golden "
+                              f"table = the GPU's; {inserts} inserts of the full trees (the oracle's exact "
+                              f"counts); {wall:.1f} s wall, {sum(secs):.1f} process-seconds, longest map "
+                              f"{max(secs):.1f} s"}
         jobs.append(ref_job)
     return rec, jobs
 
@@ -545,11 +578,8 @@ def main():
         if name in SYM_WORKLOADS:
             rec, ex = sym_workload(args, mf, rl, cost, name, st, 1, model, pairs, losses, train_s)
             if not args.no_cpu:
-                ref_win = REF_WINDOWS[name]
-                c3, m3 = mf.sym_cases(ex["n"], ex["level"], ref_win[1], ref_win[2])
-                r3, _, _ = mf.sym_solve(ex["n"], ex["level"], c3, m3, mf.SolveOptions(count_events=True))
-                cpu_jobs.append((name, sym_ref_job(name, ex["n"], ex["level"], r3.count * ex["n"] ** 2, r3.nodes,
-                                                   threads)))
+                cpu_jobs.append((name, sym_ref_job(REF_EXE[ex["n"]], ex["n"], ex["level"], ex["start"],
+                                                   ex["n_inputs"], ex["total"], ex["nodes"], threads)))
                 k = PORT_CASES[name]
                 cpu_jobs.append((name + ":port", sym_port_job(ex["n"], ex["level"], ex["cyc"][:k], ex["mul"][:k],
                                                               threads)))
@@ -564,11 +594,8 @@ def main():
     h = out["headline"]
     ex = out["ex"]
     if ex is not None and not args.no_cpu:
-        ref_win = REF_WINDOWS[head]
-        c3, m3 = mf.sym_cases(ex["n"], ex["level"], ref_win[1], ref_win[2])
-        r3, _, _ = mf.sym_solve(ex["n"], ex["level"], c3, m3, mf.SolveOptions(count_events=True))
-        cpu_jobs.insert(0, (head, sym_ref_job(head, ex["n"], ex["level"], r3.count * ex["n"] ** 2, r3.nodes,
-                                              threads)))
+        cpu_jobs.insert(0, (head, sym_ref_job(REF_EXE[ex["n"]], ex["n"], ex["level"], ex["start"], ex["n_inputs"],
+                                              ex["total"], ex["nodes"], threads)))
         k = PORT_CASES[head]
         cpu_jobs.insert(1, (head + ":port", sym_port_job(ex["n"], ex["level"], ex["cyc"][:k], ex["mul"][:k],
                                                          threads)))
@@ -600,6 +627,13 @@ def main():
         if r is not None:
             log(f"cpu baseline {cname}: {r['value']:.3e} {r['unit']} on {r['cores']} threads "
                 f"({time.perf_counter() - t1:.1f} s)")
+    # While the plain kernel still runs, the oracle-port sample of the
+    # headline window grows by further blocks of cases.
+    hp = cpu.get(head + ":port")
+    while th is not None and th.is_alive() and hp is not None and hp["cases"] < ex["n_inputs"]:
+        k, blk = hp["cases"], PORT_CASES[head]
+        extend_port(hp, sym_port_job(ex["n"], ex["level"], ex["cyc"][k:k + blk], ex["mul"][k:k + blk], threads)())
+        log(f"cpu baseline {head}:port: {hp['value']:.3e} nodes/s over the first {hp['cases']} cases")
     if th is not None:
         th.join()
         rp, pw = plain_box["res"], plain_box["wall"]

@@ -1289,20 +1289,35 @@ int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t
             int rc = check_expanded(pr, expanded);
             if (rc != MF_OK) return rc;
         }
-        // Stream the window in batches that bound host and device memory.
+        // Stream the window in batches that bound host and device memory; a
+        // host thread generates batch k+1 while the GPU counts batch k.
         const uint64_t batch = 1ull << 22;
         uint64_t pos = start, left = max_cases == 0 ? ~0ull : max_cases;
-        std::vector<uint8_t> cyc;
-        std::vector<uint64_t> mul;
+        struct Batch {
+            std::vector<uint8_t> cyc;
+            std::vector<uint64_t> mul;
+            uint64_t want = 0;
+            int64_t got = 0;
+        } cur, nxt;
+        auto generate = [&pr, level, seed_start, seed_count](Batch &b, uint64_t at, uint64_t want) {
+            b.want = want;
+            b.cyc.resize(want * pr.case_bytes());
+            b.mul.resize(want);
+            b.got = sym_cases(pr, level, at, want, b.cyc.data(), b.mul.data(), seed_start, seed_count);
+        };
         unsigned __int128 sum = 0;
         mf_result part;
+        generate(cur, pos, std::min(batch, left));
         while (left > 0) {
-            const uint64_t want = std::min(batch, left);
-            cyc.resize(want * pr.case_bytes());
-            mul.resize(want);
-            const int64_t got = sym_cases(pr, level, pos, want, cyc.data(), mul.data(), seed_start, seed_count);
+            const int64_t got = cur.got;
             if (got <= 0) break;
-            int rc = mf_sym_solve(n, level, cyc.data(), mul.data(), got, opt, nullptr, nullptr, &part);
+            const bool more = static_cast<uint64_t>(got) == cur.want && left > static_cast<uint64_t>(got);
+            std::thread producer;
+            if (more)
+                producer = std::thread(generate, std::ref(nxt), pos + static_cast<uint64_t>(got),
+                                       std::min(batch, left - static_cast<uint64_t>(got)));
+            int rc = mf_sym_solve(n, level, cur.cyc.data(), cur.mul.data(), got, opt, nullptr, nullptr, &part);
+            if (producer.joinable()) producer.join();
             if (rc != MF_OK) return rc;
             sum += (static_cast<unsigned __int128>(part.count_hi) << 64) | part.count_lo;
             out->nodes += part.nodes;
@@ -1315,7 +1330,8 @@ int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t
             out->span_ms += part.span_ms;
             pos += static_cast<uint64_t>(got);
             left -= static_cast<uint64_t>(got);
-            if (static_cast<uint64_t>(got) < want) break;
+            if (!more) break;
+            std::swap(cur, nxt);
         }
         ans = sum * static_cast<unsigned>(n * n);
     }

@@ -38,6 +38,63 @@ def run_chunk(n: int, threads: int, expanded_n: int, start_case: int, max_cases:
     }
 
 
+def gen_chunk(n: int, expanded_n: int, start_case: int, max_cases: int):
+    """Host side of a chunk: its cases from the stream index (cycles, mults, seconds)."""
+    import map_folding_amd as mf
+    t0 = time.perf_counter()
+    cyc, mul = mf.sym_cases(n, expanded_n, start_case, max_cases)
+    return cyc, mul, time.perf_counter() - t0
+
+
+def solve_chunk(n: int, threads: int, expanded_n: int, start_case: int, max_cases: int, cases,
+                schedule: dict | None = None) -> dict:
+    """GPU side of a chunk whose cases `gen_chunk` made: upload, (learned:
+    features + inference on the device, per-case deepening targets), solve.
+    `schedule` holds deepen / model / theta / cap (all optional)."""
+    import map_folding_amd as mf
+    cyc, mul, gen_s = cases
+    schedule = schedule or {}
+    t0 = time.perf_counter()
+    front = mf.SymFrontier(n, expanded_n, cyc, mul)
+    try:
+        levels = None
+        deepen = schedule.get("deepen")
+        if schedule.get("model") is not None:
+            from map_folding_amd import cost
+            levels = cost.learned_levels(schedule["model"].predict_frontier(front), deepen,
+                                         schedule["theta"], schedule["cap"])
+        res, _, _ = front.solve(mf.SolveOptions(deepen=deepen if deepen is not None else 0, deepen_levels=levels))
+    finally:
+        front.close()
+    wall_s = time.perf_counter() - t0
+    count = res.count * n * n
+    return {
+        "n": n, "threads": threads, "expanded_n": expanded_n, "start_case": start_case, "max_cases": max_cases,
+        "count": str(count), "wall_s": wall_s, "completed_hint": max_cases,
+        "stderr_tail": f"kernel_ms={res.kernel_ms:.1f} rounds={res.rounds} nodes={res.nodes} "
+                       f"host_generation_s={gen_s:.4f} (overlapped with the previous chunk) "
+                       f"schedule={'learned' if levels is not None else 'uniform'}",
+    }
+
+
+def pipelined_chunks(inputs, schedule: dict | None = None):
+    """Rows of the chunks in order, the host generating chunk k+1 on a worker
+    thread while the GPU counts chunk k (the generator and the solve both run
+    in the native library, outside the interpreter lock)."""
+    import concurrent.futures as cf
+    if not inputs:
+        return
+    with cf.ThreadPoolExecutor(max_workers=1) as pool:
+        n, _, e, s, m = inputs[0]
+        pending = pool.submit(gen_chunk, n, e, s, m)
+        for k, item in enumerate(inputs):
+            cases = pending.result()
+            if k + 1 < len(inputs):
+                n1, _, e1, s1, m1 = inputs[k + 1]
+                pending = pool.submit(gen_chunk, n1, e1, s1, m1)
+            yield solve_chunk(*item, cases, schedule)
+
+
 def summarize_jsonl(path: Path, total_cases: int, chunk_size: int) -> dict:
     """Coverage and sums of a results file (modal_sym.py:36-77)."""
     rows = []
@@ -76,8 +133,13 @@ def summarize_jsonl(path: Path, total_cases: int, chunk_size: int) -> dict:
 
 def sweep(n: int = 8, threads: int = 16, expanded_n: int = 36, total_cases: int = TOTAL_8X8_DEPTH36_CASES,
           chunk_size: int = 100000, start_case: int = 0, chunk_count: int = 0, out: str = "",
-          resume_existing: bool = False, log=print) -> dict:
-    """Run chunks in order and append one row per chunk (modal_sym.py:152-232)."""
+          resume_existing: bool = False, log=print, pipeline: bool = True, schedule: dict | None = None) -> dict:
+    """Run chunks in order and append one row per chunk (modal_sym.py:152-232).
+
+    With `pipeline` (n >= 7, where chunks are windows of the expanded-case
+    stream) the next chunk's cases are generated while the current one is
+    counted; `schedule` as for solve_chunk.  Otherwise each chunk is one
+    run_chunk call (generation, then the solve)."""
     if chunk_size <= 0:
         raise ValueError("chunk_size must be positive")
     starts = list(range(start_case, total_cases, chunk_size))
@@ -110,9 +172,9 @@ def sweep(n: int = 8, threads: int = 16, expanded_n: int = 36, total_cases: int 
     total = 0
     started = time.perf_counter()
     log(f"chunks={len(inputs)} chunk_size={chunk_size} out={out_path}")
+    rows = (pipelined_chunks(inputs, schedule) if pipeline and n >= 7 else (run_chunk(*item) for item in inputs))
     with out_path.open("a", encoding="utf-8") as f:
-        for k, item in enumerate(inputs, 1):
-            row = run_chunk(*item)
+        for k, row in enumerate(rows, 1):
             total += int(row["count"])
             f.write(json.dumps(row, sort_keys=True) + "\n")
             f.flush()
@@ -121,7 +183,7 @@ def sweep(n: int = 8, threads: int = 16, expanded_n: int = 36, total_cases: int 
     summary = {
         "chunks": len(inputs), "chunk_size": chunk_size, "start_case": start_case, "total_cases": total_cases,
         "expanded_n": expanded_n, "partial_sum": str(total), "elapsed_s": time.perf_counter() - started,
-        "out": str(out_path),
+        "out": str(out_path), "pipelined": bool(pipeline and n >= 7),
     }
     out_path.with_suffix(out_path.suffix + ".summary.json").write_text(
         json.dumps(summary, indent=2, sort_keys=True) + "\n", encoding="utf-8")
@@ -157,6 +219,11 @@ def main(argv=None) -> int:
     s.add_argument("--chunk-count", type=int, default=0)
     s.add_argument("--out", default="")
     s.add_argument("--resume-existing", action="store_true")
+    s.add_argument("--no-pipeline", action="store_true", help="generate each chunk only after the previous solve")
+    s.add_argument("--deepen", type=int, default=None, help="deepening target (default: the library's)")
+    s.add_argument("--cost-params", default="", help="cost-model parameters (.npy): learned per-case targets")
+    s.add_argument("--theta", type=float, default=21.0, help="learned threshold (log2 predicted nodes)")
+    s.add_argument("--cap", type=int, default=4, help="most extra levels the model may add")
     v.add_argument("--out", required=True)
     v.add_argument("--total-cases", type=int, default=TOTAL_8X8_DEPTH36_CASES)
     v.add_argument("--chunk-size", type=int, default=50000)
@@ -166,8 +233,16 @@ def main(argv=None) -> int:
         print(json.dumps(run_chunk(a.n, a.threads, a.expanded_n, a.start_case, a.max_cases), indent=2,
                          sort_keys=True))
     elif a.cmd == "sweep":
+        sched = {"deepen": a.deepen}
+        if a.cost_params:
+            from map_folding_amd import cost
+            model = cost.CostModel(seed=0)
+            model.load(a.cost_params)
+            if a.deepen is None:
+                raise SystemExit("--cost-params needs --deepen (the base the learned targets add to)")
+            sched.update(model=model, theta=a.theta, cap=a.cap)
         sweep(a.n, a.threads, a.expanded_n, a.total_cases, a.chunk_size, a.start_case, a.chunk_count, a.out,
-              a.resume_existing, log=lambda m: print(m, flush=True))
+              a.resume_existing, log=lambda m: print(m, flush=True), pipeline=not a.no_pipeline, schedule=sched)
     else:
         print(json.dumps(validate(a.out, a.total_cases, a.chunk_size, a.expected), indent=2, sort_keys=True))
     return 0

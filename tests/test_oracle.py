@@ -65,6 +65,25 @@ def test_lunnon_small_rectangles_and_transpose(oracle, golden):
     assert oracle.lunnon_count([1, 5])[0] == golden["strips"]["1x5"]
 
 
+def test_rectangle_insert_table(oracle, golden):
+    """The full-tree insert counts the rectangle CPU baseline divides by its
+    wall time: one entry per golden shape, the small ones recomputed."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "rect_inserts.json")) as f:
+        ins = json.load(f)["inserts"]
+    shapes = {k for k in golden["rectangles"] if not k.startswith("_")}
+    assert set(ins) == shapes
+    for shape in shapes:
+        p, q = map(int, shape.split("x"))
+        if p * q <= 20:
+            assert oracle.lunnon_count([p, q])[1]["inserts"] == ins[shape], shape
+    # More leaves, more inserts along each family.
+    for p in (2, 3, 4):
+        row = sorted((int(k.split("x")[1]), v) for k, v in ins.items() if k.startswith(f"{p}x"))
+        assert all(a[1] < b[1] for a, b in zip(row, row[1:]))
+
+
 def test_lunnon_5x5_event_counts(oracle, golden):
     total, st = oracle.lunnon_count([5, 5])
     assert total == golden["square_totals"]["5"]
