@@ -237,7 +237,8 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     # Tuner: deepening target, learned threshold, budget.
     t1 = time.perf_counter()
     if args.no_tune or n > 8:
-        cfg = {"deepen": DEEPENS[n][1] if len(DEEPENS[n]) > 1 else -1, "theta": None, "cap": None, "budget": 64}
+        cfg = {"deepen": args.deepen if args.deepen is not None else (DEEPENS[n][1] if len(DEEPENS[n]) > 1 else -1),
+               "theta": args.theta if pred is not None else None, "cap": args.cap, "budget": args.budget}
     else:
         cfg = autotune.tune_sym(front, pred=pred, level=level, deepens=DEEPENS[n], log=log)
     cfg["tune_s"] = time.perf_counter() - t1
@@ -249,7 +250,7 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     for w in range(warmup):
         r, _, _ = front.solve(opts)
         log(f"{name}: warmup {w}: {r.total_ms:.1f} ms")
-    wall, kern, rounds, items = [], [], [], []
+    wall, kern, rounds, items, inputs = [], [], [], [], []
     for k in range(steps):
         t1 = time.perf_counter()
         r, _, _ = front.solve(opts)
@@ -259,6 +260,7 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
         kern.append(r.kernel_ms / 1e3)
         rounds.append(r.rounds)
         items.append(r.items)
+        inputs.append(r.inputs)
         log(f"{name}: step {k}: wall {wall[-1]*1e3:.1f} ms, kernels {r.kernel_ms:.1f} ms, rounds {r.rounds}, "
             f"items {r.items}")
     ms = 1e3 * sum(wall) / len(wall)
@@ -266,10 +268,11 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     ops = rl.sym_ops(res.nodes, res.walks, res.steps)
     # The same schedule without the model (uniform deepening) and without deepening.
     comp = {}
-    for label, o in (("no_model", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
-                                                  waves_per_cu=32, deepen=cfg["deepen"])),
-                     ("undeepened", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
-                                                    waves_per_cu=32, deepen=-1))):
+    variants = (("no_model", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
+                                             waves_per_cu=32, deepen=cfg["deepen"])),
+                ("undeepened", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
+                                               waves_per_cu=32, deepen=-1)))
+    for label, o in [] if args.no_compare else variants:
         if label == "no_model" and levels is None:
             comp[label] = {"ms": ms, "kernel_ms": kernel_s * 1e3, "rounds": rounds[-1], "same_as_learned": True}
             continue
@@ -288,7 +291,7 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     # End to end for one chunk: generation (warm index) + [features + inference]
     # + upload + solve.
     e2e = {}
-    for label in (["learned", "no_model"] if levels is not None else ["no_model"]):
+    for label in [] if args.no_compare else (["learned", "no_model"] if levels is not None else ["no_model"]):
         t1 = time.perf_counter()
         c2, m2 = mf.sym_cases(n, level, start, cases)
         t_gen = time.perf_counter() - t1
@@ -319,7 +322,8 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
            "ms_per_step": ms, "kernel_ms_per_step": kernel_s * 1e3, "nodes": res.nodes, "page_walks": res.walks,
            "walk_steps": res.steps, "leaves": res.leaves, "nodes_per_s": res.nodes / (ms / 1e3),
            "cases_per_s": cyc.shape[0] / (ms / 1e3), "rounds": rounds[-1] if rounds else None,
-           "items_per_step": items[-1] if items else None, "ops_per_step": ops,
+           "items_per_step": items[-1] if items else None,
+           "first_round_items": inputs[-1] if inputs else None, "ops_per_step": ops,
            "achieved_tops": ops / kernel_s / 1e12,
            "schedule": "persistent re-split, device deepening" + ("" if levels is None else " + learned per-case depth"),
            "tuned": {k: cfg[k] for k in ("deepen", "theta", "cap", "budget", "tune_s")},
@@ -524,7 +528,15 @@ def main():
     ap.add_argument("--cases", type=int, default=None, help="headline chunk size (default 50,000)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain static-frontier kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-tune", action="store_true", help="fixed knobs instead of the tuner")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="fixed knobs instead of the tuner (sym: --deepen/--theta/--cap/--budget)")
+    ap.add_argument("--deepen", type=int, default=None, help="sym with --no-tune: deepening target (-1 off)")
+    ap.add_argument("--theta", type=float, default=None,
+                    help="sym with --no-tune: learned threshold (log2 predicted nodes); unset = no learned targets")
+    ap.add_argument("--cap", type=int, default=4, help="sym with --no-tune: most levels the model adds")
+    ap.add_argument("--budget", type=int, default=64, help="sym with --no-tune: re-split budget")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="sym: skip the no-model / undeepened / end-to-end runs (profiling)")
     ap.add_argument("--cost-params", default="",
                     help="load cost-model parameters (.npy) instead of training them on the GPU")
     ap.add_argument("--seed", type=int, default=0, help="rect64: seed of the shape draw")
@@ -651,7 +663,10 @@ def main():
     for rec in [h] + configs:
         rec["roofline_frac"] = rec["achieved_tops"] / peak
     if ex is not None:
-        bytes_per_launch = (ex["item_bytes"] * (2 * h["items_per_step"] - ex["n_inputs"]) / h["rounds"]
+        # Every item a round handles is read once; all but the first round's
+        # (written by the deepening pass or the upload) were written once by
+        # the round that split them.
+        bytes_per_launch = (ex["item_bytes"] * (2 * h["items_per_step"] - h["first_round_items"]) / h["rounds"]
                             if h["items_per_step"] else None)
         result = {
             "metric": "DFS nodes/s & cases/s on 1 MI355X, batched+learned vs plain, bit-exact",
@@ -664,6 +679,7 @@ def main():
             "cases_per_s": h["cases_per_s"], "nodes": h["nodes"], "page_walks": h["page_walks"],
             "walk_steps": h["walk_steps"], "leaves": h["leaves"], "rounds": h["rounds"],
             "kernel_ms_per_step": h["kernel_ms_per_step"], "items_per_step": h["items_per_step"],
+            "first_round_items": h["first_round_items"],
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "roofline": {"bound": "valu", "achieved": h["achieved_tops"], "peak": peak, "unit": "TOP/s",
                          "frac": h["roofline_frac"], "traffic": _traffic(head, *(SYM_WORKLOADS[head][2:4]))

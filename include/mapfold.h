@@ -78,6 +78,8 @@ typedef struct {
     double kernel_ms;      /* sum of the solve kernels' durations (HIP events per launch) */
     double span_ms;        /* device time from the first launch to the end of the last */
     double total_ms;       /* host wall time of the call */
+    uint64_t inputs;       /* items the first round started from (after any device deepening);
+                              items - inputs of them were written by the re-split rounds */
 } mf_result;
 
 /* Frozen int32-op weights of the algorithmic work model (roofline figure):

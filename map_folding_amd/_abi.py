@@ -33,6 +33,7 @@ class MFResult(C.Structure):
         ("leaves", C.c_uint64), ("cands", C.c_uint64), ("items", C.c_uint64),
         ("rounds", C.c_uint32), ("status", C.c_uint32),
         ("kernel_ms", C.c_double), ("span_ms", C.c_double), ("total_ms", C.c_double),
+        ("inputs", C.c_uint64),
     ]
 
     @property

@@ -214,6 +214,7 @@ int run_solve(Args args, const void *kstatic, const void *kpersist, uint32_t lds
     uint32_t rounds = 0;
     double launch_ms = 0.0;
     if (num > 0) {
+        out->inputs = num;
         if (t.schedule == MF_SCHED_STATIC) {
             if ((rc = set_lds_limit(kstatic, lds_bytes)) != MF_OK) return rc;
             args.items = dev_items;
@@ -1326,6 +1327,7 @@ int mf_sym_fold_slice(int32_t n, int32_t expanded, uint64_t seed_start, uint64_t
             out->leaves += part.leaves;
             out->items += part.items;
             out->rounds += part.rounds;
+            out->inputs += part.inputs;
             out->kernel_ms += part.kernel_ms;
             out->span_ms += part.span_ms;
             pos += static_cast<uint64_t>(got);

@@ -242,6 +242,7 @@ extern "C" int mf_sym_solve_noalloc(int32_t n, int32_t level, const uint8_t *cyc
     out->leaves = h[ACC_LEAVES];
     out->items = h[ACC_ITEMS];
     out->rounds = rounds;
+    out->inputs = static_cast<uint64_t>(num);
     float ms = 0.0f;
     NA_TRY(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
     out->kernel_ms = ms;
