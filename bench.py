@@ -238,15 +238,17 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     t1 = time.perf_counter()
     if args.no_tune or n > 8:
         cfg = {"deepen": args.deepen if args.deepen is not None else (DEEPENS[n][1] if len(DEEPENS[n]) > 1 else -1),
-               "theta": args.theta if pred is not None else None, "cap": args.cap, "budget": args.budget}
+               "theta": args.theta if pred is not None else None, "cap": args.cap, "budget": args.budget,
+               "lanes": 2 if (args.lanes == 2 and n <= 8) else 1, "trials": []}
     else:
         cfg = autotune.tune_sym(front, pred=pred, level=level, deepens=DEEPENS[n], log=log)
     cfg["tune_s"] = time.perf_counter() - t1
     levels = None if cfg["theta"] is None else cost.learned_levels(pred, cfg["deepen"], cfg["theta"], cfg["cap"])
+    core = 3 if cfg.get("lanes", 1) == 2 else 0
     log(f"{name}: tuned deepen {cfg['deepen']} theta {cfg['theta']} cap {cfg['cap']} budget {cfg['budget']} "
-        f"({cfg['tune_s']:.1f} s)")
+        f"lanes per subtree {cfg.get('lanes', 1)} ({cfg['tune_s']:.1f} s)")
     opts = mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"], waves_per_cu=32,
-                           deepen=cfg["deepen"], deepen_levels=levels)
+                           deepen=cfg["deepen"], deepen_levels=levels, core=core)
     for w in range(warmup):
         r, _, _ = front.solve(opts)
         log(f"{name}: warmup {w}: {r.total_ms:.1f} ms")
@@ -269,9 +271,9 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     # The same schedule without the model (uniform deepening) and without deepening.
     comp = {}
     variants = (("no_model", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
-                                             waves_per_cu=32, deepen=cfg["deepen"])),
+                                             waves_per_cu=32, deepen=cfg["deepen"], core=core)),
                 ("undeepened", mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
-                                               waves_per_cu=32, deepen=-1)))
+                                               waves_per_cu=32, deepen=-1, core=core)))
     for label, o in [] if args.no_compare else variants:
         if label == "no_model" and levels is None:
             comp[label] = {"ms": ms, "kernel_ms": kernel_s * 1e3, "rounds": rounds[-1], "same_as_learned": True}
@@ -307,7 +309,7 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
             t_model = time.perf_counter() - t2
         t2 = time.perf_counter()
         r2, _, _ = f2.solve(mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"],
-                                            waves_per_cu=32, deepen=cfg["deepen"], deepen_levels=lv))
+                                            waves_per_cu=32, deepen=cfg["deepen"], deepen_levels=lv, core=core))
         t_solve = time.perf_counter() - t2
         f2.close()
         if r2.count * n * n != total:
@@ -326,7 +328,8 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
            "first_round_items": inputs[-1] if inputs else None, "ops_per_step": ops,
            "achieved_tops": ops / kernel_s / 1e12,
            "schedule": "persistent re-split, device deepening" + ("" if levels is None else " + learned per-case depth"),
-           "tuned": {k: cfg[k] for k in ("deepen", "theta", "cap", "budget", "tune_s")},
+           "tuned": {**{k: cfg.get(k, 1) for k in ("deepen", "theta", "cap", "budget", "lanes", "tune_s")},
+                     "trials": [[t["trial"], round(t["kernel_ms"], 2)] for t in cfg.get("trials", [])]},
            "learned": learned, **comp, "end_to_end_ms": e2e}
     return rec, {"n": n, "level": level, "start": start, "cyc": cyc, "mul": mul, "front": front, "total": total,
                  "nodes": res.nodes, "walks": res.walks, "steps": res.steps, "case_nodes": case_nodes,
@@ -535,6 +538,8 @@ def main():
                     help="sym with --no-tune: learned threshold (log2 predicted nodes); unset = no learned targets")
     ap.add_argument("--cap", type=int, default=4, help="sym with --no-tune: most levels the model adds")
     ap.add_argument("--budget", type=int, default=64, help="sym with --no-tune: re-split budget")
+    ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
+                    help="sym with --no-tune: lanes per subtree (2: a lane pair splits each node's page walks)")
     ap.add_argument("--no-compare", action="store_true",
                     help="sym: skip the no-model / undeepened / end-to-end runs (profiling)")
     ap.add_argument("--cost-params", default="",

@@ -103,7 +103,10 @@ typedef struct {
                                  drained before it splits its remaining work */
     int32_t count_events;     /* 1: count walks/steps too (nodes/leaves always counted) */
     int32_t core;             /* sym DFS stack: 0 = default (compact), 1 = one u64 mask per
-                                 level, 2 = compact 16-bit entries with recompute */
+                                 level, 2 = compact 16-bit entries with recompute, 3 = compact
+                                 with two lanes per subtree (a lane pair shares one row and
+                                 stack and splits each node's page walks; re-split schedule,
+                                 up to 8x8) */
     int32_t graph;            /* re-split: 0 = default (on), 1 = on, 2 = off.  On: after the
                                  first round, rounds run as a captured HIP graph of 8 rounds
                                  (device-side item counts), replayed until a round spills

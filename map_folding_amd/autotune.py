@@ -134,27 +134,30 @@ def tune_resplit(frontier, budgets=(32, 64, 128), tail_mult=(1, 4), waves=(20, 3
 
 
 def tune_sym(frontier, pred=None, level=None, deepens=(45, 46, 47, 48), thetas=(19, 21, 23), caps=(2, 4),
-             budgets=(64, 128), waves=32, repeats=1, log=None):
+             budgets=(64, 128), lanes=(1, 2), waves=32, repeats=1, log=None):
     """The symmetry DFS's partition depth and split threshold on a resident
     case frontier: the device deepening target (every case expanded to that
     level before the rounds: fewer stack levels per lane, more resident
     waves), then -- given the cost model's predictions `pred` (log2 nodes per
     case) -- the learned threshold theta and the cap on extra levels for the
-    cases predicted large (cost.learned_levels), then the re-split budget.
-    Each stage keeps the best of the previous one.  `level` is the cases'
-    level (deepening targets at or below it are skipped).
+    cases predicted large (cost.learned_levels), then the re-split budget,
+    then the lanes per subtree (1, or 2: core 3, a lane pair splitting each
+    node's page walks).  Each stage keeps the best of the previous one.
+    `level` is the cases' level (deepening targets at or below it are
+    skipped).
 
-    Returns {"deepen", "theta", "cap", "budget", "kernel_ms", "trials"};
-    theta/cap are None when the learned levels did not beat the uniform target."""
+    Returns {"deepen", "theta", "cap", "budget", "lanes", "kernel_ms",
+    "trials"}; theta/cap are None when the learned levels did not beat the
+    uniform target."""
     from . import cost
     say = log or (lambda m: None)
     trials = []
     ref = None
 
-    def trial(name, deepen, levels, budget):
+    def trial(name, deepen, levels, budget, core=0):
         nonlocal ref
         opt = SolveOptions(count_events=False, budget=budget, tail=budget, waves_per_cu=waves, deepen=deepen,
-                           deepen_levels=levels)
+                           deepen_levels=levels, core=core)
 
         def once():
             r, _, _ = frontier.solve(opt)
@@ -164,7 +167,7 @@ def tune_sym(frontier, pred=None, level=None, deepens=(45, 46, 47, 48), thetas=(
             ref = res
         elif res != ref:
             raise RuntimeError("a tuning trial changed the count")
-        trials.append({"trial": name, "deepen": deepen, "budget": budget, "kernel_ms": ms})
+        trials.append({"trial": name, "deepen": deepen, "budget": budget, "core": core, "kernel_ms": ms})
         say(f"  {name}: {ms:.1f} ms")
         return ms
 
@@ -191,7 +194,14 @@ def tune_sym(frontier, pred=None, level=None, deepens=(45, 46, 47, 48), thetas=(
         ms = trial(f"budget {bu}", deepen, levels, bu)
         if ms < best_ms:
             best_ms, budget = ms, bu
-    return {"deepen": deepen, "theta": theta, "cap": cap, "budget": budget, "kernel_ms": best_ms, "trials": trials}
+    nlanes = 1
+    for ln in lanes:
+        if ln == 2:
+            ms = trial("two lanes per subtree", deepen, levels, budget, core=3)
+            if ms < best_ms:
+                best_ms, nlanes = ms, 2
+    return {"deepen": deepen, "theta": theta, "cap": cap, "budget": budget, "lanes": nlanes, "kernel_ms": best_ms,
+            "trials": trials}
 
 
 def save(cfg: dict, path: str) -> None:

@@ -128,7 +128,7 @@ constexpr int kDefaultCore = 2;   // compact 16-bit stack
 Tuned tune(const mf_solve_options *opt) {
     Tuned t;
     t.schedule = opt ? opt->schedule : MF_SCHED_RESPLIT;
-    t.core = (opt && (opt->core == 1 || opt->core == 2)) ? opt->core : kDefaultCore;
+    t.core = (opt && opt->core >= 1 && opt->core <= 3) ? opt->core : kDefaultCore;
     t.block = (opt && opt->block > 0) ? static_cast<uint32_t>(opt->block) : 256u;
     t.block = std::max(64u, std::min(256u, (t.block / 64u) * 64u));
     t.waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
@@ -593,6 +593,8 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
     if (f->kind == 1 || f->kind == 3) {
         const bool c96 = f->kind == 3;
         const int core = c96 ? 2 : t.core;
+        if (core == 3 && t.schedule != MF_SCHED_RESPLIT)
+            return fail(MF_ERR_ARG, "core 3 (two lanes per subtree) runs the re-split schedule only");
         SymArgs a;
         uint32_t levels = 0;
         sym_args(*f, a, &levels);
@@ -637,9 +639,10 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
                 const WorkItem *deep = nullptr;
                 uint64_t deep_num = 0;
                 double deep_ms = 0.0;
-                rc = deepen_frontier(a, core, t, cells, dev, f->num, origins, per_count != nullptr, per_nodes != nullptr,
-                                     static_cast<uint32_t>(target), per_item ? item_targets.data() : nullptr, &deep,
-                                     &deep_num, &deep_ms);
+                // The deepening pass runs one lane per item (the compact-stack instance).
+                rc = deepen_frontier(a, core == 3 ? 2 : core, t, cells, dev, f->num, origins, per_count != nullptr,
+                                     per_nodes != nullptr, static_cast<uint32_t>(target),
+                                     per_item ? item_targets.data() : nullptr, &deep, &deep_num, &deep_ms);
                 if (rc == MF_OK) {
                     SymArgs a2 = a;
                     // Every deepened item is at or past the uniform target

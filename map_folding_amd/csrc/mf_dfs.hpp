@@ -168,11 +168,12 @@ struct WideStack {
 // 2 state bits: 0 = j is the last child, 1 = the next sibling is the last,
 // 2 = more follow the next sibling, 3 = more follow j but are not recorded
 // (the node is recomputed to find them).  B = 6 for up to 64 cells (u64
-// masks), 7 for up to 128 (u128 masks).
-template <typename Mask, uint32_t B>
+// masks), 7 for up to 128 (u128 masks).  A level's slot holds one entry per
+// stack of the wave: 64, or 32 when two lanes share a subtree.
+template <typename Mask, uint32_t B, uint32_t kStacks = 64>
 struct CompactStackT {
     static_assert(2 * B + 2 <= 16, "entry must fit 16 bits");
-    static constexpr uint32_t kSlotBytes = 128;   // u16 x 64 lanes
+    static constexpr uint32_t kSlotBytes = 2 * kStacks;   // u16 x stacks
     static constexpr uint32_t kLaneBytes = 2;
     static constexpr uint32_t kField = (1u << B) - 1u;
     using Entry = uint32_t;

@@ -64,9 +64,12 @@ namespace {
 #define MF_SYM_ROW_STRIDE 68
 #endif
 template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_,
-          uint32_t kRowStride_ = MF_SYM_ROW_STRIDE>
+          uint32_t kRowStride_ = MF_SYM_ROW_STRIDE, uint32_t kLanes_ = 1>
 struct Geometry {
     static constexpr uint32_t kCells = kCells_;
+    // Lanes per subtree: 1, or 2 (a lane pair shares one row and one stack,
+    // keeps identical DFS state, and splits each node's page walks).
+    static constexpr uint32_t kLanes = kLanes_;
     static constexpr uint32_t kRowCells = kRowCells_;   // entries of a lane's LDS row (cells of the largest map)
     static constexpr uint32_t kRowStride = kRowStride_;   // bytes from entry c to entry c+1 of a lane
     using Mask = Mask_;
@@ -100,12 +103,17 @@ using G96 = Geometry<96, unsigned __int128, WorkItem128, CompactStack128, 81, 64
 // Up to 7x7: 49-entry rows (3,136 B per wave): six 4-wave blocks per CU at
 // 7x7 depth 24 instead of five.
 using G49 = Geometry<64, uint64_t, WorkItem, CompactStack, 49>;
+// Two lanes per subtree, up to 8x8: 32 rows per wave at a 36-byte stride (9
+// dwords, odd) and 32 stacks, half the LDS of G64 per wave.
+using G64Pair = Geometry<64, uint64_t, WorkItem, CompactStackT<uint64_t, 6, 32>, 64, 36, 2>;
 
-// The geometry of a launch: `cells` = n*n of the map.
-enum class Geo { k49, k64, k64Wide, k96 };
+// The geometry of a launch: `cells` = n*n of the map; core 3 = two lanes per
+// subtree (re-split schedule, up to 8x8).
+enum class Geo { k49, k64, k64Wide, k96, k64Pair };
 inline Geo geometry(int core, uint32_t cells) {
     if (cells > 64) return Geo::k96;
     if (core == 1) return Geo::k64Wide;
+    if (core == 3) return Geo::k64Pair;
     return cells <= 49 ? Geo::k49 : Geo::k64;
 }
 
@@ -162,6 +170,7 @@ struct Work {
     uint32_t isteps;   // wrap, fold_item below)
     bool have;
     bool popping;      // backtracking continues this step (no node to expand)
+    bool primary;      // the lane that counts and spills for its subtree (lanes per subtree > 1)
 };
 
 struct Totals {
@@ -240,6 +249,39 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
     // (bit - 7).
     const uint32_t ew = reinterpret_cast<const uint32_t *>(lds + G::kPartnerOff)[i];
     uint32_t act = ew & 0x80808080u;
+    if constexpr (G::kLanes == 2) {
+        // Two lanes per subtree: per round the even lane walks the first of
+        // the next two pages, the odd lane the second (or the first again
+        // when there is no second), each to its own chain's end; the pair
+        // swaps reaches and step counts.  Both lanes hold the same state, so
+        // both run every round and the swap always finds its partner.
+        const bool odd = (threadIdx.x & 1u) != 0u;
+        while (act != 0 && good != 0) {
+            const uint32_t b1 = static_cast<uint32_t>(__builtin_ctz(act));
+            act &= act - 1u;
+            uint32_t b2 = b1;
+            const bool two = act != 0;
+            if (two) {
+                b2 = static_cast<uint32_t>(__builtin_ctz(act));
+                act &= act - 1u;
+            }
+            const uint32_t b = odd ? b2 : b1;
+            uint32_t st = 0;
+            const Mask mine = page_walk<G>(lds, cl, i, page_row<G>(b >> 3), (ew >> (b - 7u)) & 0x7fu, st);
+            const Mask other = __shfl_xor(mine, 1, 64);
+            const uint32_t ost = static_cast<uint32_t>(__shfl_xor(static_cast<int>(st), 1, 64));
+            // Event counts as the one-page-at-a-time order.
+            good &= odd ? other : mine;
+            ++walks;
+            steps += odd ? ost : st;
+            if (two && good != 0) {
+                ++walks;
+                steps += odd ? st : ost;
+            }
+            good &= odd ? mine : other;
+        }
+        return good;
+    }
     if constexpr (walk_mode<G>() == 2) {
         while (act != 0 && good != 0) {
             const uint32_t b1 = static_cast<uint32_t>(__builtin_ctz(act));
@@ -441,6 +483,7 @@ __device__ __forceinline__ void load_item(uint8_t *lds, const Lane &ln, const ty
 // Fold the current item's counters into the lane totals and the per-origin
 // results (when an item finishes and when it is split).
 __device__ __forceinline__ void retire(const SymArgs &a, const Work &w, Totals &t) {
+    if (!w.primary) return;
     t.sum += static_cast<unsigned long long>(w.mult) * w.leaves;
     t.nodes += w.inodes;
     t.walks += w.iwalks;
@@ -459,6 +502,7 @@ __device__ __forceinline__ void fold_item(const SymArgs &a, Work &w, Totals &t) 
     FoldTotals f;
     uint32_t no_cands = 0;
     const uint32_t moved = fold_item_counters(w.inodes, w.iwalks, w.isteps, no_cands, f);
+    if (!w.primary) return;   // the partner lane folded the same counts the same way
     t.nodes += f.nodes;
     t.walks += f.walks;
     t.steps += f.steps;
@@ -480,12 +524,12 @@ __device__ __forceinline__ void flush_wave(const SymArgs &a, uint32_t lane, cons
 
 template <typename G>
 __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) {
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, slot = (tid & 63u) / G::kLanes, wave = tid >> 6;
     const uint32_t wave_bytes = G::kCycleBytes + levels * G::Stack::kSlotBytes;
     const uint32_t wbase = G::kTableBytes + wave * wave_bytes;
     Lane ln;
-    ln.cl = row_lane_base(wbase, lane);
-    ln.sl = wbase + G::kCycleBytes + lane * G::Stack::kLaneBytes;
+    ln.cl = row_lane_base(wbase, slot);
+    ln.sl = wbase + G::kCycleBytes + slot * G::Stack::kLaneBytes;
     ln.base = base_level;
     ln.s0 = ln.sl - base_level * G::Stack::kSlotBytes;
     return ln;
@@ -526,7 +570,7 @@ __device__ __forceinline__ bool spill_all(const SymArgs &a, uint8_t *lds, const 
     // Items this lane emits: the pending node (unless it is mid-backtrack,
     // when levels >= i are finished), then every level with siblings left.
     uint32_t need = 0;
-    if (w.have) {
+    if (w.have && w.primary) {
         need = w.popping ? 0u : 1u;
         for (uint32_t t = w.root; t < w.i; ++t) need += Stack::more(Stack::get(lds, ln, t)) != 0u ? 1u : 0u;
     }
@@ -539,7 +583,7 @@ __device__ __forceinline__ bool spill_all(const SymArgs &a, uint8_t *lds, const 
     uint32_t t = w.i;
     typename G::Mask mask = 0;
     uint32_t floor = w.floor;   // the pending expansion may itself be a recompute
-    bool emit = pend && !w.popping;
+    bool emit = pend && !w.popping && w.primary;
     const unsigned long long below_lane = (1ull << lane) - 1ull;
     while (__any(pend)) {
         const unsigned long long m = __ballot(emit);
@@ -570,7 +614,7 @@ __device__ __forceinline__ bool spill_all(const SymArgs &a, uint8_t *lds, const 
                     mask = 0;
                     floor = j + 1;
                 }
-                emit = mask != 0 || floor != 0;
+                emit = (mask != 0 || floor != 0) && w.primary;
             }
         }
     }
@@ -590,11 +634,13 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
     stage_tables<G>(a, lds);
     const Lane ln = make_lane<G>(a.levels, a.base_level);
     const auto *items = static_cast<const typename G::Item *>(a.items);
+    static_assert(G::kLanes == 1, "the static schedule runs one lane per item");
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     Totals tot;
     Work w;
     w.have = false;
     w.popping = false;
+    w.primary = true;
     if (idx < a.num_items) load_item<G>(lds, ln, items + idx, w);
     while (w.have) {
         if (dfs_step<G, EV>(lds, ln, a.last, w)) {
@@ -633,22 +679,24 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
     Work w;
     w.have = false;
     w.popping = false;
+    w.primary = (lane % G::kLanes) == 0u;
     bool drained = false;   // wave-uniform: the input array has been handed out
     uint32_t tail_left = a.tail;
 
     for (;;) {
         if (!drained) {
             // Idle lanes claim the next input items, one atomic per wave.
+            // (The lanes of a subtree are idle together: one claim per subtree.)
             const unsigned long long idle = __ballot(!w.have);
             if (idle != 0) {
-                const uint32_t want = static_cast<uint32_t>(__popcll(idle));
+                const uint32_t want = static_cast<uint32_t>(__popcll(idle)) / G::kLanes;
                 const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(idle));
                 uint32_t got = 0;
                 if (lane == first) got = atomicAdd(a.claim, want);
                 got = __shfl(got, first, 64);
                 if (got + want >= num_items) drained = true;
                 if (!w.have) {
-                    const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below));
+                    const uint32_t idx = got + static_cast<uint32_t>(__popcll(idle & below)) / G::kLanes;
                     if (idx < num_items) load_item<G>(lds, ln, items + idx, w);
                 }
             } else {
@@ -916,6 +964,10 @@ const void *pick(bool persistent, bool events) {
 // cells = n*n picks the row size (49, 64 or, for 9x9, 81 entries).
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells) {
     switch (geometry(core, cells)) {
+        case Geo::k64Pair:   // re-split schedule only
+            if (!persistent) return nullptr;
+            return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, true>)
+                          : reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, false>);
         case Geo::k96: return pick<G96>(persistent, events);
         case Geo::k64Wide: return pick<G64Wide>(persistent, events);
         case Geo::k49: return pick<G49>(persistent, events);
@@ -964,6 +1016,7 @@ const void *sym_deepen_ptr(int core, bool events, uint32_t cells) {
 
 uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells) {
     switch (geometry(core, cells)) {
+        case Geo::k64Pair: return lds_of<G64Pair>(levels, block);
         case Geo::k96: return lds_of<G96>(levels, block);
         case Geo::k64Wide: return lds_of<G64Wide>(levels, block);
         case Geo::k49: return lds_of<G49>(levels, block);

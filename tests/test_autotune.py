@@ -44,3 +44,41 @@ def test_tune_resplit_keeps_count(gpu, golden):
     r, _, _ = f.solve(gpu.SolveOptions(budget=cfg["budget"], tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"]))
     assert r.count == ref.count
     f.close()
+
+
+class _FakeSymFrontier:
+    """Stands in for a resident frontier: a fixed count, and a kernel time
+    that depends on the options as `ms_of` says."""
+
+    def __init__(self, ms_of):
+        self.ms_of = ms_of
+        self.seen = []
+
+    def solve(self, opt):
+        import types
+        self.seen.append(opt)
+        return types.SimpleNamespace(kernel_ms=self.ms_of(opt), count=7, nodes=11), None, None
+
+
+def test_tune_sym_stages_and_lanes():
+    """Each stage keeps the best of the previous one; the lanes stage times
+    core 3 at the chosen schedule and keeps it only when it is faster."""
+    import numpy as np
+    from map_folding_amd import autotune
+
+    def ms(opt):
+        t = 100.0 - (5.0 if opt.deepen == 46 else 0.0) - (2.0 if opt.deepen_levels is not None else 0.0)
+        return t + (30.0 if opt.core == 3 else 0.0) + (1.0 if opt.budget == 128 else 0.0)
+    f = _FakeSymFrontier(ms)
+    pred = np.array([10.0, 25.0, 30.0])
+    cfg = autotune.tune_sym(f, pred=pred, level=36, deepens=(45, 46), thetas=(21,), caps=(4,), budgets=(64, 128))
+    assert (cfg["deepen"], cfg["theta"], cfg["cap"], cfg["budget"], cfg["lanes"]) == (46, 21, 4, 64, 1)
+    assert cfg["trials"][-1]["trial"] == "two lanes per subtree" and cfg["trials"][-1]["core"] == 3
+    assert f.seen[-1].core == 3 and f.seen[-1].deepen == 46 and f.seen[-1].deepen_levels is not None
+    # A faster pair kernel is kept.
+    f2 = _FakeSymFrontier(lambda o: ms(o) - (60.0 if o.core == 3 else 0.0))
+    assert autotune.tune_sym(f2, pred=pred, level=36, deepens=(46,), thetas=(21,), caps=(4,))["lanes"] == 2
+    # Without the pair stage, no core-3 trial runs.
+    f3 = _FakeSymFrontier(ms)
+    assert autotune.tune_sym(f3, level=36, deepens=(46,), lanes=(1,))["lanes"] == 1
+    assert all(o.core == 0 for o in f3.seen)

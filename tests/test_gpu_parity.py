@@ -28,6 +28,9 @@ SCHEDULES = {
     # host-driven rounds instead of the captured HIP graph of rounds
     "resplit_nograph": dict(schedule=1, graph=2),
     "resplit_tight_nograph": dict(schedule=1, budget=8, tail=8, graph=2),
+    # two lanes per subtree (sym, up to 8x8; re-split only)
+    "resplit_pair": dict(schedule=1, core=3),
+    "resplit_pair_tight": dict(schedule=1, core=3, budget=4, tail=4),
 }
 LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight", "resplit_tight_nograph"]
 
@@ -138,7 +141,8 @@ def _chunk(golden, n, e, start, mx):
 
 
 @pytest.mark.parametrize("sched", ["resplit", "resplit_tight", "resplit_tiny", "resplit_wide",
-                                   "resplit_wide_tight", "resplit_nograph", "resplit_tight_nograph"])
+                                   "resplit_wide_tight", "resplit_nograph", "resplit_tight_nograph",
+                                   "resplit_pair", "resplit_pair_tight"])
 def test_sym_8x8_first_300_events(gpu, golden, sched):
     _say(f"sym 8x8 d36 [0,300) schedule={sched}")
     g = _chunk(golden, 8, 36, 0, 300)
@@ -168,7 +172,8 @@ def test_sym_per_case_parity_vs_oracle(gpu, oracle):
         cyc, mul = _small_cases(gpu, n, e, start, k, 400000)
         assert cyc.shape[0] >= 20
         cnt, nodes, tot, st = oracle.sym_solve(n, e, cyc, mul)
-        for sched in ("static", "resplit_tight", "resplit_tiny", "static_wide", "resplit_wide_tight"):
+        for sched in ("static", "resplit_tight", "resplit_tiny", "static_wide", "resplit_wide_tight",
+                      "resplit_pair_tight"):
             res, pc, pn = gpu.sym_solve(n, e, cyc, mul, _opts(gpu, sched), per_item=True)
             assert np.array_equal(pc, cnt), sched
             assert np.array_equal(pn, nodes), sched
@@ -379,3 +384,23 @@ def test_per_item_deepening_targets_keep_counts(gpu):
     with pytest.raises(gpu.MFError):
         f.solve(_opts(gpu, "resplit", deepen_levels=lv[:10]))
     f.close()
+
+
+def test_two_lanes_per_subtree_keep_every_count(gpu):
+    """Core 3 (a lane pair per subtree, the pair splitting each node's page
+    walks) gives the per-case counts, per-case nodes and exact event counters
+    of the one-lane kernel, deepened or not; it runs the re-split schedule
+    only."""
+    for n, e, start, num in ((7, 24, 200000, 3000), (8, 36, 120000000, 1500)):
+        _say(f"two lanes per subtree: {n}x{n} level {e} [{start},+{num})")
+        cyc, mul = gpu.sym_cases(n, e, start, num)
+        f = gpu.SymFrontier(n, e, cyc, mul)
+        ref, pc0, pn0 = f.solve(_opts(gpu, "resplit", deepen=-1), per_item=True)
+        for sched, deepen in (("resplit_pair", -1), ("resplit_pair", 0), ("resplit_pair_tight", -1)):
+            r, pc, pn = f.solve(_opts(gpu, sched, deepen=deepen), per_item=True)
+            assert (r.count, r.nodes, r.walks, r.steps, r.leaves) == (ref.count, ref.nodes, ref.walks, ref.steps,
+                                                                      ref.leaves), (sched, deepen)
+            assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0), (sched, deepen)
+        with pytest.raises(gpu.MFError):
+            f.solve(_opts(gpu, "static", core=3))
+        f.close()
