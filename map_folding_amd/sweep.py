@@ -29,7 +29,8 @@ def run_chunk(n: int, threads: int, expanded_n: int, start_case: int, max_cases:
     """One chunk on the GPU (modal_sym.py:88-141 run_chunk, without the subprocess)."""
     import map_folding_amd as mf
     t0 = time.perf_counter()
-    count, res = mf.sym_fold(n, expanded=expanded_n, start=start_case, max_cases=max_cases)
+    count, res = mf.sym_fold(n, expanded=expanded_n, start=start_case, max_cases=max_cases,
+                             options=mf.SolveOptions(count_events=False))
     wall_s = time.perf_counter() - t0
     return {
         "n": n, "threads": threads, "expanded_n": expanded_n, "start_case": start_case, "max_cases": max_cases,
@@ -63,7 +64,8 @@ def solve_chunk(n: int, threads: int, expanded_n: int, start_case: int, max_case
             from map_folding_amd import cost
             levels = cost.learned_levels(schedule["model"].predict_frontier(front), deepen,
                                          schedule["theta"], schedule["cap"])
-        res, _, _ = front.solve(mf.SolveOptions(deepen=deepen if deepen is not None else 0, deepen_levels=levels))
+        res, _, _ = front.solve(mf.SolveOptions(count_events=False, deepen=deepen if deepen is not None else 0,
+                                                deepen_levels=levels))
     finally:
         front.close()
     wall_s = time.perf_counter() - t0
@@ -136,10 +138,10 @@ def sweep(n: int = 8, threads: int = 16, expanded_n: int = 36, total_cases: int 
           resume_existing: bool = False, log=print, pipeline: bool = True, schedule: dict | None = None) -> dict:
     """Run chunks in order and append one row per chunk (modal_sym.py:152-232).
 
-    With `pipeline` (n >= 7, where chunks are windows of the expanded-case
-    stream) the next chunk's cases are generated while the current one is
-    counted; `schedule` as for solve_chunk.  Otherwise each chunk is one
-    run_chunk call (generation, then the solve)."""
+    For n >= 7 (chunks are windows of the expanded-case stream) each chunk is
+    generated (gen_chunk) and counted (solve_chunk, `schedule` as there); with
+    `pipeline` the next chunk's generation overlaps the current solve,
+    otherwise it follows it.  For n < 7 each chunk is one run_chunk call."""
     if chunk_size <= 0:
         raise ValueError("chunk_size must be positive")
     starts = list(range(start_case, total_cases, chunk_size))
@@ -172,7 +174,12 @@ def sweep(n: int = 8, threads: int = 16, expanded_n: int = 36, total_cases: int 
     total = 0
     started = time.perf_counter()
     log(f"chunks={len(inputs)} chunk_size={chunk_size} out={out_path}")
-    rows = (pipelined_chunks(inputs, schedule) if pipeline and n >= 7 else (run_chunk(*item) for item in inputs))
+    if n < 7:
+        rows = (run_chunk(*item) for item in inputs)
+    elif pipeline:
+        rows = pipelined_chunks(inputs, schedule)
+    else:
+        rows = (solve_chunk(*item, gen_chunk(item[0], item[2], item[3], item[4]), schedule) for item in inputs)
     with out_path.open("a", encoding="utf-8") as f:
         for k, row in enumerate(rows, 1):
             total += int(row["count"])

@@ -12,8 +12,18 @@ def _fake_chunk(n, threads, expanded_n, start, count):
             "count": str(1000 + start), "wall_s": 0.0, "completed_hint": count, "stderr_tail": ""}
 
 
+def _fake_gen(n, expanded_n, start, count):
+    return ("cases", start, count), None, 0.0
+
+
+def _fake_solve(n, threads, expanded_n, start, count, cases, schedule=None):
+    assert cases[0] == ("cases", start, count)
+    return _fake_chunk(n, threads, expanded_n, start, count)
+
+
 def test_sweep_resume_validate(tmp_path, monkeypatch):
-    monkeypatch.setattr(sweep, "run_chunk", _fake_chunk)
+    monkeypatch.setattr(sweep, "gen_chunk", _fake_gen)
+    monkeypatch.setattr(sweep, "solve_chunk", _fake_solve)
     out = tmp_path / "r.jsonl"
     logs = []
     s = sweep.sweep(n=7, expanded_n=24, total_cases=95, chunk_size=20, chunk_count=2, out=str(out), log=logs.append,
