@@ -230,12 +230,14 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
     using Mask = typename G::Mask;
     Mask reach = static_cast<Mask>(1) << anchor;
     uint32_t j = cyc_ld<G>(lds, cl, anchor);
+    __builtin_amdgcn_s_setprio(1);   // as in the paired walk (good_mask)
     while (j != anchor) {
         ++steps;
         j = hop<G>(prow, i, j);
         reach |= static_cast<Mask>(1) << j;
         j = cyc_ld<G>(lds, cl, j);
     }
+    __builtin_amdgcn_s_setprio(0);
     return reach;
 }
 
@@ -309,6 +311,11 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             // hops from the anchor to itself: the same.
             uint32_t s1 = 0, s2 = 0;
             bool o1 = j1 != a1, o2 = j2 != a2;
+            // Waves inside a walk loop issue first: their next instruction
+            // is usually waiting on an LDS load, so handing them the issue
+            // slot as soon as it lands keeps more loads in flight (1.3% less
+            // time on the 8x8 chunk; priority 1 and 3 measure the same).
+            __builtin_amdgcn_s_setprio(1);
             do {
 #pragma unroll
                 for (int u = 0; u < MF_SYM_PAIR_UNROLL; ++u) {
@@ -328,6 +335,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                     j2 = o2 ? n2 : j2;
                 }
             } while (o1 || o2);
+            __builtin_amdgcn_s_setprio(0);
             // Event counts as the one-page-at-a-time order: the second walk
             // happens only if the first left good cells.
             good &= reach1;
