@@ -525,7 +525,7 @@ int deepen_frontier(const SymArgs &base, int core, const Tuned &t, uint32_t cell
     }
     // Room for the deepened frontier and for one level in flight (the spill
     // queues, which the rounds reuse afterwards).
-    const uint64_t cap = std::min<uint64_t>(0xffffffffull, std::max<uint64_t>(num * 8, uint64_t{24} << 20));
+    const uint64_t cap = std::min<uint64_t>(0xffffffffull, std::max<uint64_t>(num * 8, uint64_t{32} << 20));
     if ((rc = g.deep.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
     if ((rc = g.spill_a.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
     if ((rc = g.spill_b.ensure(cap * sizeof(WorkItem))) != MF_OK) return rc;
@@ -640,9 +640,18 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
                 uint64_t deep_num = 0;
                 double deep_ms = 0.0;
                 // The deepening pass runs one lane per item (the compact-stack instance).
-                rc = deepen_frontier(a, core == 3 ? 2 : core, t, cells, dev, f->num, origins, per_count != nullptr,
+                const int dcore = core == 3 ? 2 : core;
+                rc = deepen_frontier(a, dcore, t, cells, dev, f->num, origins, per_count != nullptr,
                                      per_nodes != nullptr, static_cast<uint32_t>(target),
                                      per_item ? item_targets.data() : nullptr, &deep, &deep_num, &deep_ms);
+                if (rc == kDeepenFull && per_item && target > f->level) {
+                    // The per-item targets outgrew the buffers: deepen every
+                    // item to the uniform target alone (the pass starts over,
+                    // clearing what the first attempt counted).
+                    rc = deepen_frontier(a, dcore, t, cells, dev, f->num, origins, per_count != nullptr,
+                                         per_nodes != nullptr, static_cast<uint32_t>(target), nullptr, &deep,
+                                         &deep_num, &deep_ms);
+                }
                 if (rc == MF_OK) {
                     SymArgs a2 = a;
                     // Every deepened item is at or past the uniform target

@@ -367,6 +367,22 @@ def test_deepening_out_of_room_falls_back(gpu):
     assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0)
 
 
+def test_per_item_targets_out_of_room_fall_back_to_uniform(gpu):
+    """Per-item targets that outgrow the deepening buffers give way to the
+    uniform target (not to an undeepened solve): the first round then starts
+    from the uniformly deepened frontier, and every count is unchanged."""
+    _say("per-item deepening targets out of room: 8x8 level 36 [0,+300)")
+    cyc, mul = gpu.sym_cases(8, 36, 0, 300)
+    f = gpu.SymFrontier(8, 36, cyc, mul)
+    ref, pc0, pn0 = f.solve(_opts(gpu, "resplit", deepen=-1), per_item=True)
+    uni, _, _ = f.solve(_opts(gpu, "resplit", deepen=44), per_item=True)
+    r, pc, pn = f.solve(_opts(gpu, "resplit", deepen=44, deepen_levels=np.full(300, 61, np.uint8)), per_item=True)
+    assert (r.count, r.nodes, r.walks, r.steps) == (ref.count, ref.nodes, ref.walks, ref.steps)
+    assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0)
+    assert ref.inputs == 300 and uni.inputs > 300 and r.inputs == uni.inputs
+    f.close()
+
+
 def test_per_item_deepening_targets_keep_counts(gpu):
     """Per-item deepening targets (as the cost model picks them), with and
     without an item order, give the counts of the undeepened solve."""
