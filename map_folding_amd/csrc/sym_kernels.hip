@@ -663,7 +663,7 @@ __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
 
 // ------------------------------------------------------------- persistent
 template <typename G, bool EV>
-__global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
+__device__ __forceinline__ void persistent_body(const SymArgs &a) {
     extern __shared__ __align__(16) uint8_t lds[];
     // Graph rounds read their item count from the previous round's spill
     // counter; an empty round returns at once (uniformly for the grid).
@@ -741,6 +741,19 @@ __global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
         }
     }
     flush_wave(a, lane, tot);
+}
+
+template <typename G, bool EV>
+__global__ __launch_bounds__(256) void sym_persistent_kernel(SymArgs a) {
+    persistent_body<G, EV>(a);
+}
+
+// Up to 7x7 the rows are short enough that LDS holds 7 blocks per CU once
+// the cases are deepened; the register budget is capped to match (at most
+// 72 VGPRs, 7 waves per SIMD).
+template <bool EV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void sym_persistent_kernel49(SymArgs a) {
+    persistent_body<G49, EV>(a);
 }
 
 // ------------------------------------------------------------- deepening
@@ -978,7 +991,11 @@ const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cell
                           : reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, false>);
         case Geo::k96: return pick<G96>(persistent, events);
         case Geo::k64Wide: return pick<G64Wide>(persistent, events);
-        case Geo::k49: return pick<G49>(persistent, events);
+        case Geo::k49:
+            if (persistent)
+                return events ? reinterpret_cast<const void *>(&sym_persistent_kernel49<true>)
+                              : reinterpret_cast<const void *>(&sym_persistent_kernel49<false>);
+            return pick<G49>(false, events);
         default: return pick<G64>(persistent, events);
     }
 }
