@@ -206,6 +206,9 @@ struct Totals {
 #ifndef MF_SYM_PAIR_UNROLL
 #define MF_SYM_PAIR_UNROLL 2
 #endif
+#ifndef MF_SYM_STICKY_WALK
+#define MF_SYM_STICKY_WALK 1
+#endif
 template <typename G>
 constexpr int walk_mode() {
     return MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE : (sizeof(typename G::Mask) > 8 ? 1 : 2);
@@ -304,6 +307,38 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 j2 = cyc_ld<G>(lds, cl, a2);
                 reach2 = static_cast<Mask>(1) << a2;
             }
+#if MF_SYM_STICKY_WALK
+            // A chain that closes walks its orbit again (back through the
+            // anchor: the anchor's partner on this page is cell i, not yet
+            // placed, so the walk re-enters the orbit), which only re-adds
+            // cells already in reach.  Whether a chain is still open is a
+            // lane mask that only ever loses lanes (scalar ANDs), so the
+            // loop carries no per-step select.  A walk visits each cell of
+            // its orbit once, so its step count is |reach| - 1.
+            bool o1 = j1 != a1, o2 = j2 != a2;
+            __builtin_amdgcn_s_setprio(1);
+            do {
+#pragma unroll
+                for (int u = 0; u < MF_SYM_PAIR_UNROLL; ++u) {
+                    const uint32_t h1 = hop<G>(r1, i, j1);
+                    const uint32_t h2 = hop<G>(r2, i, j2);
+                    j1 = cyc_ld<G>(lds, cl, h1);
+                    j2 = cyc_ld<G>(lds, cl, h2);
+                    // The loaded bytes are already zero-extended: keep the
+                    // compiler from narrowing the loop-carried cells to 8
+                    // bits (which costs a mask per step).
+                    asm("" : "+v"(j1));
+                    asm("" : "+v"(j2));
+                    reach1 |= static_cast<Mask>(1) << h1;
+                    reach2 |= static_cast<Mask>(1) << h2;
+                    o1 = o1 && j1 != a1;
+                    o2 = o2 && j2 != a2;
+                }
+            } while (o1 || o2);
+            __builtin_amdgcn_s_setprio(0);
+            const uint32_t s1 = EVC ? mask_popcount(reach1) - 1u : 0u;
+            const uint32_t s2 = EVC ? mask_popcount(reach2) - 1u : 0u;
+#else
             // A chain that closes keeps its last cell j: stepping it again
             // hops to the same cell (already in reach) and back to the
             // anchor, so a closed chain needs no mask.  A walk of no steps
@@ -336,6 +371,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 }
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
+#endif
             // Event counts as the one-page-at-a-time order: the second walk
             // happens only if the first left good cells.
             good &= reach1;
