@@ -53,7 +53,8 @@ namespace {
 // to 96 (9x9: 81 cells, u128 masks, 128-byte items).  LDS: a partner word per
 // cell (bits 8p..8p+6 = partner on page p, n2 = none; bit 8p+7 set when that
 // partner is below the cell, i.e. get_good walks page p for it), the
-// partners page-major (byte kPageOff + kCells*p + c), then per wave a
+// partners page-major (byte kPageOff + kCells*p + c), a level word per cell
+// (the pages its get_good walks, level_word below), then per wave a
 // kCells-entry byte row per lane and the stack.
 // Row stride of the u64 geometries: 68 bytes (17 dwords, odd) instead of 64
 // puts the 4 lanes of a quad in different banks when they read different
@@ -77,7 +78,8 @@ struct Geometry {
     using Stack = Stack_;
     static constexpr uint32_t kPartnerOff = 0;
     static constexpr uint32_t kPageOff = 4 * kCells;
-    static constexpr uint32_t kTableBytes = 8 * kCells;
+    static constexpr uint32_t kLevelOff = 8 * kCells;
+    static constexpr uint32_t kTableBytes = 12 * kCells;
     // Every lane's row, rounded so that the stack after it stays 16-byte aligned.
     static constexpr uint32_t kCycleBytes = (kRowCells * kRowStride + 15u) & ~15u;
     static constexpr uint32_t kStateBytes = kCells;
@@ -186,13 +188,13 @@ struct Totals {
 //      (ascending, so the walks and their early exit are unchanged), and the
 //      wave runs max(#pages) rounds; the page offset becomes a per-lane
 //      value (12% less time than mode 0 on the 8x8 depth-36 chunk);
-//   2  pairs: a round walks two of the lane's pages in one loop.  Both
-//      chains step every iteration, a closed one in place, so that the two
-//      chains' LDS loads are in flight together: a walk step is two
-//      dependent LDS loads, and at 5 waves per SIMD their latency, not VALU
-//      issue, bounds the loop.  A closed chain repeats its last step, which
-//      changes nothing, so it needs no mask: 9 VALU per chain step.  13%
-//      less time than mode 1 on the 8x8 chunk.
+//   2  pairs: the level's two pages (every cell past the spiral's first few
+//      walks one or two; the level table names them) walk in one loop, both
+//      chains stepping every iteration, so that their LDS loads are in
+//      flight together: a walk step is two dependent LDS loads.  A closed
+//      chain walks its orbit again under a scalar open mask, which adds
+//      nothing to its reach: 7 VALU per chain step.  13% less time than
+//      mode 1 on the 8x8 chunk.
 // The default is pairs for u64 masks and rounds for u128 masks (9x9), where
 // two 128-bit bit-sets per iteration cost more than the overlap saves (5.6%
 // slower than rounds; 4% with closed chains unmasked).  MF_SYM_WALK_MODE
@@ -205,9 +207,6 @@ struct Totals {
 // chunks than every step; three per trip is slower again).
 #ifndef MF_SYM_PAIR_UNROLL
 #define MF_SYM_PAIR_UNROLL 2
-#endif
-#ifndef MF_SYM_STICKY_WALK
-#define MF_SYM_STICKY_WALK 1
 #endif
 template <typename G>
 constexpr int walk_mode() {
@@ -242,6 +241,38 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
     }
     __builtin_amdgcn_s_setprio(0);
     return reach;
+}
+
+// The pages get_good walks at level i, as one word: bits 0-6 the first
+// walked page's anchor (its partner of cell i), bit 7 set when a second page
+// is walked, bits 8-14 the second anchor, bits 16-23 / 24-31 the two pages'
+// partner-row offsets from kPageOff.  With one page the second repeats the
+// first.  kLevelGeneric: no page, more than two, or rows past a byte (9x9):
+// the walk takes the page-by-page path.
+constexpr uint32_t kLevelGeneric = 0x8000u;
+template <typename G>
+__device__ __forceinline__ uint32_t level_word(uint32_t partners, uint32_t i) {
+    uint32_t n = 0, a0 = 0, a1 = 0, r0 = 0, r1 = 0;
+#pragma unroll
+    for (uint32_t p = 0; p < 4; ++p) {
+        const uint32_t o = (partners >> (8u * p)) & 0xffu;
+        if (o < i) {
+            if (n == 0) {
+                a0 = o;
+                r0 = G::kCells * p;
+            } else if (n == 1) {
+                a1 = o;
+                r1 = G::kCells * p;
+            }
+            ++n;
+        }
+    }
+    if (n == 0 || n > 2 || 3u * G::kCells > 255u) return kLevelGeneric;
+    if (n == 1) {
+        a1 = a0;
+        r1 = r0;
+    }
+    return a0 | (n == 2 ? 0x80u : 0u) | (a1 << 8) | (r0 << 16) | (r1 << 24);
 }
 
 template <typename G, bool EVC>
@@ -288,34 +319,27 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
         return good;
     }
     if constexpr (walk_mode<G>() == 2) {
-        while (act != 0 && good != 0) {
-            const uint32_t b1 = static_cast<uint32_t>(__builtin_ctz(act));
-            act &= act - 1u;
-            const uint32_t a1 = (ew >> (b1 - 7u)) & 0x7fu, r1 = page_row<G>(b1 >> 3);
-            uint32_t j1 = cyc_ld<G>(lds, cl, a1);
-            Mask reach1 = static_cast<Mask>(1) << a1;
-            // No second page: the second chain repeats the first (same cells,
-            // same reach, not counted), which leaves `good` unchanged.
-            uint32_t a2 = a1, r2 = r1, j2 = j1;
-            Mask reach2 = reach1;
-            const bool two = act != 0;
-            if (two) {
-                const uint32_t b2 = static_cast<uint32_t>(__builtin_ctz(act));
-                act &= act - 1u;
-                a2 = (ew >> (b2 - 7u)) & 0x7fu;
-                r2 = page_row<G>(b2 >> 3);
-                j2 = cyc_ld<G>(lds, cl, a2);
-                reach2 = static_cast<Mask>(1) << a2;
-            }
-#if MF_SYM_STICKY_WALK
+        // Pairs: both pages of the level in one loop (every level past the
+        // spiral's first few walks one or two pages; others take the rounds
+        // below).
+        const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[i];
+        if ((lw & kLevelGeneric) == 0u) {
+            const uint32_t a1 = lw & 0x7fu, a2 = (lw >> 8) & 0x7fu;
+            const uint32_t r1 = G::kPageOff + ((lw >> 16) & 0xffu), r2 = G::kPageOff + (lw >> 24);
+            uint32_t j1 = cyc_ld<G>(lds, cl, a1), j2 = cyc_ld<G>(lds, cl, a2);
+            Mask reach1 = static_cast<Mask>(1) << a1, reach2 = static_cast<Mask>(1) << a2;
             // A chain that closes walks its orbit again (back through the
             // anchor: the anchor's partner on this page is cell i, not yet
             // placed, so the walk re-enters the orbit), which only re-adds
             // cells already in reach.  Whether a chain is still open is a
-            // lane mask that only ever loses lanes (scalar ANDs), so the
-            // loop carries no per-step select.  A walk visits each cell of
-            // its orbit once, so its step count is |reach| - 1.
+            // lane mask that only ever loses lanes (scalar ANDs), so the loop
+            // carries no per-step select.  A walk visits each cell of its
+            // orbit once, so its step count is |reach| - 1.  With one page
+            // the second chain repeats the first.
             bool o1 = j1 != a1, o2 = j2 != a2;
+            // Waves inside a walk loop issue first: their next instruction is
+            // usually waiting on an LDS load, so handing them the issue slot
+            // as soon as it lands keeps more loads in flight.
             __builtin_amdgcn_s_setprio(1);
             do {
 #pragma unroll
@@ -336,56 +360,21 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 }
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
-            const uint32_t s1 = EVC ? mask_popcount(reach1) - 1u : 0u;
-            const uint32_t s2 = EVC ? mask_popcount(reach2) - 1u : 0u;
-#else
-            // A chain that closes keeps its last cell j: stepping it again
-            // hops to the same cell (already in reach) and back to the
-            // anchor, so a closed chain needs no mask.  A walk of no steps
-            // (the anchor alone in the cycle, so its partner is not placed)
-            // hops from the anchor to itself: the same.
-            uint32_t s1 = 0, s2 = 0;
-            bool o1 = j1 != a1, o2 = j2 != a2;
-            // Waves inside a walk loop issue first: their next instruction
-            // is usually waiting on an LDS load, so handing them the issue
-            // slot as soon as it lands keeps more loads in flight (1.3% less
-            // time on the 8x8 chunk; priority 1 and 3 measure the same).
-            __builtin_amdgcn_s_setprio(1);
-            do {
-#pragma unroll
-                for (int u = 0; u < MF_SYM_PAIR_UNROLL; ++u) {
-                    if (EVC) {
-                        s1 += o1 ? 1u : 0u;
-                        s2 += o2 ? 1u : 0u;
-                    }
-                    const uint32_t h1 = hop<G>(r1, i, j1);
-                    const uint32_t h2 = hop<G>(r2, i, j2);
-                    const uint32_t n1 = cyc_ld<G>(lds, cl, h1);
-                    const uint32_t n2 = cyc_ld<G>(lds, cl, h2);
-                    reach1 |= static_cast<Mask>(1) << h1;
-                    reach2 |= static_cast<Mask>(1) << h2;
-                    o1 = n1 != a1;
-                    o2 = n2 != a2;
-                    j1 = o1 ? n1 : j1;
-                    j2 = o2 ? n2 : j2;
-                }
-            } while (o1 || o2);
-            __builtin_amdgcn_s_setprio(0);
-#endif
-            // Event counts as the one-page-at-a-time order: the second walk
-            // happens only if the first left good cells.
-            good &= reach1;
-            ++walks;
-            steps += s1;
-            if (two && good != 0) {
+            const Mask first = reach1 & G::below(i);
+            if (EVC) {
+                // Event counts as the one-page-at-a-time order: the second
+                // walk happens only if the first left good cells.
                 ++walks;
-                steps += s2;
+                steps += mask_popcount(reach1) - 1u;
+                if ((lw & 0x80u) != 0u && first != 0) {
+                    ++walks;
+                    steps += mask_popcount(reach2) - 1u;
+                }
             }
-            good &= reach2;
+            return first & reach2;
         }
-        return good;
     }
-    if constexpr (walk_mode<G>() == 1) {
+    if constexpr (walk_mode<G>() >= 1) {
         while (act != 0 && good != 0) {
             const uint32_t b = static_cast<uint32_t>(__builtin_ctz(act));
             act &= act - 1u;
@@ -595,6 +584,7 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
             flagged |= (((w >> (8u * p)) & 0xffu) < tid ? 0x80u : 0u) << (8u * p);
         }
         reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = flagged;
+        reinterpret_cast<uint32_t *>(lds + G::kLevelOff)[tid] = level_word<G>(w, tid);
     }
     __syncthreads();
 }
