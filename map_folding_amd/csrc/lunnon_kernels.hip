@@ -155,8 +155,7 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
         const typename Stack::Entry e = Stack::get(lds, ln, w.l);
         const uint32_t j = Stack::child(e);
         unlink(lds, ln, j, w.l);
-        const uint32_t more = Stack::more(e);
-        if (more == 1u) {
+        if (Stack::has_next(e)) {
             const typename Stack::Entry e2 = Stack::advance(e);
             Stack::put(lds, ln, w.l, e2);
             link(lds, ln, Stack::child(e2), w.l);
@@ -164,7 +163,7 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
             w.popping = false;
             return false;
         }
-        if (more == 2u) {
+        if (Stack::recompute(e)) {
             w.floor = j + 1;
             w.popping = false;
             return false;

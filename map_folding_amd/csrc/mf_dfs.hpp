@@ -155,6 +155,8 @@ struct WideStack {
     static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
     // 0 = no sibling after the current child, 1 = advance() gives it.
     static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
+    static __device__ __forceinline__ bool has_next(uint64_t e) { return (e & (e - 1)) != 0; }
+    static __device__ __forceinline__ bool recompute(uint64_t) { return false; }
     static __device__ __forceinline__ uint64_t advance(uint64_t e) { return e & (e - 1); }
     // The siblings after the current child as a mask (for splitting); never
     // unknown for this encoding.
@@ -164,22 +166,29 @@ struct WideStack {
     }
 };
 
-// 16 bits per level: bits [0, B) current child j, [B, 2B) next sibling, then
-// 2 state bits: 0 = j is the last child, 1 = the next sibling is the last,
-// 2 = more follow the next sibling, 3 = more follow j but are not recorded
-// (the node is recomputed to find them).  B = 6 for up to 64 cells (u64
-// masks), 7 for up to 128 (u128 masks).  A level's slot holds one entry per
-// stack of the wave: 64, or 32 when two lanes share a subtree.
+// 16 bits per level: bits [0, B) current child j, [B, 2B) next sibling j2,
+// bit 2B "more": children follow that are not recorded (after j2 when j2 is
+// recorded, else after j) and the node is recomputed to find them; bit 2B+1
+// "next": j2 is recorded.  So an entry below kMore has no sibling left, one in
+// [kMore, kNext) recomputes, and one at or above kNext advances to j2 -- two
+// compares decide a backtracking step.  B = 6 for up to 64 cells (u64 masks),
+// 7 for up to 128 (u128 masks).  A level's slot holds one entry per stack of
+// the wave: 64, or 32 when two lanes share a subtree.
 template <typename Mask, uint32_t B, uint32_t kStacks = 64>
 struct CompactStackT {
     static_assert(2 * B + 2 <= 16, "entry must fit 16 bits");
     static constexpr uint32_t kSlotBytes = 2 * kStacks;   // u16 x stacks
     static constexpr uint32_t kLaneBytes = 2;
     static constexpr uint32_t kField = (1u << B) - 1u;
+    static constexpr uint32_t kMore = 1u << (2 * B);
+    static constexpr uint32_t kNext = 2u << (2 * B);
     using Entry = uint32_t;
     using MaskT = Mask;
     static __device__ __forceinline__ uint32_t get(const uint8_t *lds, const Lane &ln, uint32_t level) {
-        return *reinterpret_cast<const uint16_t *>(lds + (ln.s0 + level * kSlotBytes));
+        uint32_t e = *reinterpret_cast<const uint16_t *>(lds + (ln.s0 + level * kSlotBytes));
+        // The load zero-extends: tell the compiler nothing needs masking.
+        asm("" : "+v"(e));
+        return e;
     }
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
         *reinterpret_cast<uint16_t *>(lds + (ln.s0 + level * kSlotBytes)) = static_cast<uint16_t>(e);
@@ -189,30 +198,26 @@ struct CompactStackT {
         const Mask r = g & (g - 1);
         if (r == 0) return j;
         const uint32_t j2 = mask_ctz(r);
-        const uint32_t state = (r & (r - 1)) != 0 ? 2u : 1u;
-        return j | (j2 << B) | (state << (2 * B));
+        return j | (j2 << B) | kNext | ((r & (r - 1)) != 0 ? kMore : 0u);
     }
     // The same with the number of children (popcount of g) already known.
     static __device__ __forceinline__ uint32_t encode(Mask g, uint32_t count) {
         const uint32_t j = mask_ctz(g);
         if (count == 1) return j;
         const uint32_t j2 = mask_ctz(g & (g - 1));
-        return j | (j2 << B) | ((count > 2 ? 2u : 1u) << (2 * B));
+        return j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);
     }
     static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & kField; }
+    // A recorded next sibling: advance() gives it.
+    static __device__ __forceinline__ bool has_next(uint32_t e) { return e >= kNext; }
+    // No recorded sibling but more children: recompute the node above j.
+    static __device__ __forceinline__ bool recompute(uint32_t e) { return e >= kMore && e < kNext; }
     // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.
-    static __device__ __forceinline__ uint32_t more(uint32_t e) {
-        const uint32_t s = e >> (2 * B);
-        return s == 0 ? 0u : (s == 3 ? 2u : 1u);
-    }
-    static __device__ __forceinline__ uint32_t advance(uint32_t e) {
-        const uint32_t j2 = (e >> B) & kField;
-        return j2 | ((e >> (2 * B)) == 2u ? (3u << (2 * B)) : 0u);
-    }
+    static __device__ __forceinline__ uint32_t more(uint32_t e) { return e >= kNext ? 1u : (e >= kMore ? 2u : 0u); }
+    static __device__ __forceinline__ uint32_t advance(uint32_t e) { return ((e >> B) & kField) | (e & kMore); }
     static __device__ __forceinline__ Mask rest(uint32_t e, bool &unknown) {
-        const uint32_t s = e >> (2 * B);
-        unknown = s >= 2u;
-        return s == 1u ? (static_cast<Mask>(1) << ((e >> B) & kField)) : static_cast<Mask>(0);
+        unknown = (e & kMore) != 0;
+        return (e >= kNext && !unknown) ? (static_cast<Mask>(1) << ((e >> B) & kField)) : static_cast<Mask>(0);
     }
 };
 

@@ -452,8 +452,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         const typename Stack::Entry e = Stack::get(lds, ln, w.i);
         const uint32_t j = Stack::child(e);
         unplace<G>(lds, ln, j, w.i);
-        const uint32_t more = Stack::more(e);
-        if (more == 1u) {
+        if (Stack::has_next(e)) {
             const typename Stack::Entry e2 = Stack::advance(e);
             Stack::put(lds, ln, w.i, e2);
             place<G>(lds, ln, Stack::child(e2), w.i);
@@ -461,7 +460,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             w.popping = false;
             return false;
         }
-        if (more == 2u) {
+        if (Stack::recompute(e)) {
             w.floor = j + 1;
             w.popping = false;
             return false;
