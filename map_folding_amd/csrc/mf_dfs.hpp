@@ -150,6 +150,13 @@ struct WideStack {
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint64_t e) {
         *reinterpret_cast<uint64_t *>(lds + (ln.s0 + level * kSlotBytes)) = e;
     }
+    // The same at a slot's LDS byte address.
+    static __device__ __forceinline__ uint64_t get_at(const uint8_t *lds, uint32_t addr) {
+        return *reinterpret_cast<const uint64_t *>(lds + addr);
+    }
+    static __device__ __forceinline__ void put_at(uint8_t *lds, uint32_t addr, uint64_t e) {
+        *reinterpret_cast<uint64_t *>(lds + addr) = e;
+    }
     static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
     static __device__ __forceinline__ uint64_t encode(uint64_t g, uint32_t) { return g; }
     static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
@@ -192,6 +199,15 @@ struct CompactStackT {
     }
     static __device__ __forceinline__ void put(uint8_t *lds, const Lane &ln, uint32_t level, uint32_t e) {
         *reinterpret_cast<uint16_t *>(lds + (ln.s0 + level * kSlotBytes)) = static_cast<uint16_t>(e);
+    }
+    // The same at a slot's LDS byte address.
+    static __device__ __forceinline__ uint32_t get_at(const uint8_t *lds, uint32_t addr) {
+        uint32_t e = *reinterpret_cast<const uint16_t *>(lds + addr);
+        asm("" : "+v"(e));
+        return e;
+    }
+    static __device__ __forceinline__ void put_at(uint8_t *lds, uint32_t addr, uint32_t e) {
+        *reinterpret_cast<uint16_t *>(lds + addr) = static_cast<uint16_t>(e);
     }
     static __device__ __forceinline__ uint32_t encode(Mask g) {
         const uint32_t j = mask_ctz(g);

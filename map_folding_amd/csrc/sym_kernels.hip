@@ -153,6 +153,30 @@ __device__ __forceinline__ void place(uint8_t *lds, const Lane &ln, uint32_t j, 
     cyc_st<G>(lds, ln.cl, i, nx);
     cyc_st<G>(lds, ln.cl, j, i);
 }
+// Cycle entry at an LDS byte address (from cyc_addr).
+template <typename G>
+__device__ __forceinline__ uint32_t cyc_ld_at(const uint8_t *lds, uint32_t addr) {
+    if constexpr ((G::kRowStride & (G::kRowStride - 1)) == 0) {
+        return lds[addr];
+    } else {
+        return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(addr);
+    }
+}
+template <typename G>
+__device__ __forceinline__ void cyc_st_at(uint8_t *lds, uint32_t addr, uint32_t v) {
+    if constexpr ((G::kRowStride & (G::kRowStride - 1)) == 0) {
+        lds[addr] = static_cast<uint8_t>(v);
+    } else {
+        *reinterpret_cast<__attribute__((address_space(3))) uint8_t *>(addr) = static_cast<uint8_t>(v);
+    }
+}
+// place() with the address of cell i's entry already known.
+template <typename G>
+__device__ __forceinline__ void place_at(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i, uint32_t addr_i) {
+    const uint32_t nx = cyc_ld<G>(lds, ln.cl, j);
+    cyc_st_at<G>(lds, addr_i, nx);
+    cyc_st<G>(lds, ln.cl, j, i);
+}
 // Remove cell i again (sym_fold.cpp:215): its successor goes back to j.
 template <typename G>
 __device__ __forceinline__ void unplace(uint8_t *lds, const Lane &ln, uint32_t j, uint32_t i) {
@@ -415,9 +439,17 @@ constexpr uint32_t max_pops() {
 template <typename G, bool EV>
 __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t last, Work &w) {
     using Stack = typename G::Stack;
+    constexpr uint32_t P = max_pops<G>();
+    // The cycle entry and stack slot of level i - P: those of levels i - P .. i
+    // sit at constant offsets from them, which fold into the LDS instructions.
+    // (The bases may wrap below the lane's region; only the offsets of levels
+    // the lane holds are dereferenced.)
+    const uint32_t i0 = w.i;
+    const uint32_t cb = cyc_addr<G>(ln.cl - P * G::kRowStride, i0);
+    const uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + i0 * Stack::kSlotBytes;
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
-        typename G::Mask g = good_mask<G, EV>(lds, ln.cl, w.i, nw, ns);
+        typename G::Mask g = good_mask<G, EV>(lds, ln.cl, i0, nw, ns);
         uint32_t cnt;   // children of this node
         if (w.floor == 0) {
             ++w.inodes;
@@ -426,7 +458,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
                 w.isteps += ns;
             }
             cnt = mask_popcount(g);
-            if (w.i == last) {
+            if (i0 == last) {
                 w.leaves += cnt;
                 g = 0;
             }
@@ -437,25 +469,28 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         }
         if (g != 0) {
             const typename Stack::Entry e = Stack::encode(g, cnt);
-            Stack::put(lds, ln, w.i, e);
-            place<G>(lds, ln, Stack::child(e), w.i);
-            ++w.i;
+            Stack::put_at(lds, sb + P * Stack::kSlotBytes, e);
+            place_at<G>(lds, ln, Stack::child(e), i0, cb + P * G::kRowStride);
+            w.i = i0 + 1;
             return false;
         }
     }
-    for (uint32_t k = 0; k < max_pops<G>(); ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < P; ++k) {
         if (w.i == w.root) {
             w.popping = false;
             return true;
         }
         --w.i;
-        const typename Stack::Entry e = Stack::get(lds, ln, w.i);
+        // level i0 - 1 - k
+        const uint32_t slot = sb + (P - 1 - k) * Stack::kSlotBytes, cell = cb + (P - 1 - k) * G::kRowStride;
+        const typename Stack::Entry e = Stack::get_at(lds, slot);
         const uint32_t j = Stack::child(e);
-        unplace<G>(lds, ln, j, w.i);
+        cyc_st<G>(lds, ln.cl, j, cyc_ld_at<G>(lds, cell));   // unplace
         if (Stack::has_next(e)) {
             const typename Stack::Entry e2 = Stack::advance(e);
-            Stack::put(lds, ln, w.i, e2);
-            place<G>(lds, ln, Stack::child(e2), w.i);
+            Stack::put_at(lds, slot, e2);
+            place_at<G>(lds, ln, Stack::child(e2), w.i, cell);
             ++w.i;
             w.popping = false;
             return false;
