@@ -159,6 +159,7 @@ struct WideStack {
     }
     static __device__ __forceinline__ uint64_t encode(uint64_t g) { return g; }
     static __device__ __forceinline__ uint64_t encode(uint64_t g, uint32_t) { return g; }
+    static __device__ __forceinline__ uint64_t encode_at(uint32_t, uint64_t g, uint32_t) { return g; }
     static __device__ __forceinline__ uint32_t child(uint64_t e) { return static_cast<uint32_t>(__builtin_ctzll(e)); }
     // 0 = no sibling after the current child, 1 = advance() gives it.
     static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
@@ -219,6 +220,12 @@ struct CompactStackT {
     // The same with the number of children (popcount of g) already known.
     static __device__ __forceinline__ uint32_t encode(Mask g, uint32_t count) {
         const uint32_t j = mask_ctz(g);
+        if (count == 1) return j;
+        const uint32_t j2 = mask_ctz(g & (g - 1));
+        return j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);
+    }
+    // The same with the lowest child j (ctz of g) known too.
+    static __device__ __forceinline__ uint32_t encode_at(uint32_t j, Mask g, uint32_t count) {
         if (count == 1) return j;
         const uint32_t j2 = mask_ctz(g & (g - 1));
         return j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);

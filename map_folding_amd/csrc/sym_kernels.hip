@@ -267,6 +267,24 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
     return reach;
 }
 
+// a & b & ~m, one v_bitop3_b32 per dword (truth table of (S0, S1, S2) =
+// (0xF0, 0xCC, 0xAA): 0xF0 & 0xCC & ~0xAA = 0x40).
+__device__ __forceinline__ uint64_t and_and_not(uint64_t a, uint64_t b, uint64_t m) {
+    uint32_t lo, hi;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x40"
+        : "=v"(lo)
+        : "v"(static_cast<uint32_t>(a)), "v"(static_cast<uint32_t>(b)), "v"(static_cast<uint32_t>(m)));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x40"
+        : "=v"(hi)
+        : "v"(static_cast<uint32_t>(a >> 32)), "v"(static_cast<uint32_t>(b >> 32)),
+          "v"(static_cast<uint32_t>(m >> 32)));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+[[maybe_unused]] __device__ __forceinline__ unsigned __int128 and_and_not(unsigned __int128 a, unsigned __int128 b,
+                                                         unsigned __int128 m) {
+    return a & b & ~m;
+}
+
 // The pages get_good walks at level i, as one word: bits 0-6 the first
 // walked page's anchor (its partner of cell i), bit 7 set when a second page
 // is walked, bits 8-14 the second anchor, bits 16-23 / 24-31 the two pages'
@@ -384,18 +402,19 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                 }
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
-            const Mask first = reach1 & G::below(i);
+            // cells below i reached by both walks
+            const Mask above = ~static_cast<Mask>(0) << i;
             if (EVC) {
                 // Event counts as the one-page-at-a-time order: the second
                 // walk happens only if the first left good cells.
                 ++walks;
                 steps += mask_popcount(reach1) - 1u;
-                if ((lw & 0x80u) != 0u && first != 0) {
+                if ((lw & 0x80u) != 0u && (reach1 & ~above) != 0) {
                     ++walks;
                     steps += mask_popcount(reach2) - 1u;
                 }
             }
-            return first & reach2;
+            return and_and_not(reach1, reach2, above);
         }
     }
     if constexpr (walk_mode<G>() >= 1) {
@@ -449,28 +468,26 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
     const uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + i0 * Stack::kSlotBytes;
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
-        typename G::Mask g = good_mask<G, EV>(lds, ln.cl, i0, nw, ns);
-        uint32_t cnt;   // children of this node
+        // A recompute (floor j + 1) keeps the children above j.
+        const typename G::Mask g =
+            good_mask<G, EV>(lds, ln.cl, i0, nw, ns) & (~static_cast<typename G::Mask>(0) << w.floor);
+        const uint32_t cnt = mask_popcount(g);   // children of this node
         if (w.floor == 0) {
             ++w.inodes;
             if (EV) {
                 w.iwalks += nw;
                 w.isteps += ns;
             }
-            cnt = mask_popcount(g);
-            if (i0 == last) {
-                w.leaves += cnt;
-                g = 0;
-            }
-        } else {
-            g &= ~G::below(w.floor);
-            w.floor = 0;
-            cnt = mask_popcount(g);
         }
-        if (g != 0) {
-            const typename Stack::Entry e = Stack::encode(g, cnt);
-            Stack::put_at(lds, sb + P * Stack::kSlotBytes, e);
-            place_at<G>(lds, ln, Stack::child(e), i0, cb + P * G::kRowStride);
+        w.floor = 0;
+        // The last level counts its children as leaves (it is never
+        // recomputed: its nodes enter no children).
+        const bool leaf = i0 == last;
+        if (leaf) w.leaves += cnt;
+        if (g != 0 && !leaf) {
+            const uint32_t j = mask_ctz(g);
+            Stack::put_at(lds, sb + P * Stack::kSlotBytes, Stack::encode_at(j, g, cnt));
+            place_at<G>(lds, ln, j, i0, cb + P * G::kRowStride);
             w.i = i0 + 1;
             return false;
         }
