@@ -226,12 +226,9 @@ struct Totals {
 #ifndef MF_SYM_WALK_MODE
 #define MF_SYM_WALK_MODE -1
 #endif
-// Paired steps per loop trip: a closed chain's extra steps change nothing,
-// so the exit test runs every second step (3% less time on the 8x8 and 7x7
-// chunks than every step; three per trip is slower again).
-#ifndef MF_SYM_PAIR_UNROLL
-#define MF_SYM_PAIR_UNROLL 2
-#endif
+// The paired loop runs two steps per trip: a closed chain's extra steps
+// change nothing, so the exit test runs every second step (3% less time on
+// the 8x8 and 7x7 chunks than every step; three per trip is slower again).
 template <typename G>
 constexpr int walk_mode() {
     return MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE : (sizeof(typename G::Mask) > 8 ? 1 : 2);
@@ -279,6 +276,24 @@ __device__ __forceinline__ uint64_t and_and_not(uint64_t a, uint64_t b, uint64_t
         : "v"(static_cast<uint32_t>(a >> 32)), "v"(static_cast<uint32_t>(b >> 32)),
           "v"(static_cast<uint32_t>(m >> 32)));
     return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+// a | b | c, one v_bitop3_b32 per dword (0xF0 | 0xCC | 0xAA = 0xFE): the
+// three-input OR of the bitwise class that issues at the full rate, where
+// v_or3_b32 issues at about two thirds of it on gfx950.
+__device__ __forceinline__ uint64_t or3(uint64_t a, uint64_t b, uint64_t c) {
+    uint32_t lo, hi;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xfe"
+        : "=v"(lo)
+        : "v"(static_cast<uint32_t>(a)), "v"(static_cast<uint32_t>(b)), "v"(static_cast<uint32_t>(c)));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xfe"
+        : "=v"(hi)
+        : "v"(static_cast<uint32_t>(a >> 32)), "v"(static_cast<uint32_t>(b >> 32)),
+          "v"(static_cast<uint32_t>(c >> 32)));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+[[maybe_unused]] __device__ __forceinline__ unsigned __int128 or3(unsigned __int128 a, unsigned __int128 b,
+                                                                 unsigned __int128 c) {
+    return a | b | c;
 }
 [[maybe_unused]] __device__ __forceinline__ unsigned __int128 and_and_not(unsigned __int128 a, unsigned __int128 b,
                                                          unsigned __int128 m) {
@@ -384,8 +399,11 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             // as soon as it lands keeps more loads in flight.
             __builtin_amdgcn_s_setprio(1);
             do {
+                // Two paired steps; each chain's two new cells join its
+                // reach in one three-input OR per dword.
+                Mask bits1[2], bits2[2];
 #pragma unroll
-                for (int u = 0; u < MF_SYM_PAIR_UNROLL; ++u) {
+                for (int u = 0; u < 2; ++u) {
                     const uint32_t h1 = hop<G>(r1, i, j1);
                     const uint32_t h2 = hop<G>(r2, i, j2);
                     j1 = cyc_ld<G>(lds, cl, h1);
@@ -395,11 +413,13 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                     // bits (which costs a mask per step).
                     asm("" : "+v"(j1));
                     asm("" : "+v"(j2));
-                    reach1 |= static_cast<Mask>(1) << h1;
-                    reach2 |= static_cast<Mask>(1) << h2;
+                    bits1[u] = static_cast<Mask>(1) << h1;
+                    bits2[u] = static_cast<Mask>(1) << h2;
                     o1 = o1 && j1 != a1;
                     o2 = o2 && j2 != a2;
                 }
+                reach1 = or3(reach1, bits1[0], bits1[1]);
+                reach2 = or3(reach2, bits2[0], bits2[1]);
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
             // cells below i reached by both walks
