@@ -512,34 +512,42 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             return false;
         }
     }
+    // Backtracking, one level per pass while the lane pops: remove the level's
+    // child; the first level with a recorded sibling (entered after the passes)
+    // or with unrecorded ones (a recompute) ends it, and so does the item's
+    // root.  Each pass is one region with selects inside: a lane at its root
+    // rewrites its own cycle entry unchanged instead of branching.
+    bool pop = true, fin = false, adv = false;
+    typename Stack::Entry ae = 0;
 #pragma unroll
     for (uint32_t k = 0; k < P; ++k) {
-        if (w.i == w.root) {
-            w.popping = false;
-            return true;
-        }
-        --w.i;
-        // level i0 - 1 - k
-        const uint32_t slot = sb + (P - 1 - k) * Stack::kSlotBytes, cell = cb + (P - 1 - k) * G::kRowStride;
-        const typename Stack::Entry e = Stack::get_at(lds, slot);
-        const uint32_t j = Stack::child(e);
-        cyc_st<G>(lds, ln.cl, j, cyc_ld_at<G>(lds, cell));   // unplace
-        if (Stack::has_next(e)) {
-            const typename Stack::Entry e2 = Stack::advance(e);
-            Stack::put_at(lds, slot, e2);
-            place_at<G>(lds, ln, Stack::child(e2), w.i, cell);
-            ++w.i;
-            w.popping = false;
-            return false;
-        }
-        if (Stack::recompute(e)) {
-            w.floor = j + 1;
-            w.popping = false;
-            return false;
+        if (pop) {
+            // level i0 - 1 - k
+            const uint32_t slot = sb + (P - 1 - k) * Stack::kSlotBytes, cell = cb + (P - 1 - k) * G::kRowStride;
+            const typename Stack::Entry e = Stack::get_at(lds, slot);
+            const uint32_t c = cyc_ld_at<G>(lds, cell);
+            const bool root = w.i == w.root;
+            const uint32_t j = Stack::child(e);
+            cyc_st<G>(lds, ln.cl, root ? w.i - 1 : j, c);   // unplace (at the root: unchanged)
+            const bool next = !root && Stack::has_next(e);
+            const bool rec = !root && Stack::recompute(e);
+            ae = next ? e : ae;
+            w.floor = rec ? j + 1 : w.floor;
+            w.i = root ? w.i : w.i - 1;
+            fin = root;
+            adv = next;
+            pop = !(root || next || rec);
         }
     }
-    w.popping = true;
-    return false;
+    if (adv) {
+        // enter the recorded next sibling of level w.i
+        const typename Stack::Entry e2 = Stack::advance(ae);
+        Stack::put(lds, ln, w.i, e2);
+        place<G>(lds, ln, Stack::child(e2), w.i);
+        ++w.i;
+    }
+    w.popping = pop;
+    return fin;
 }
 
 // ------------------------------------------------------------ item plumbing
