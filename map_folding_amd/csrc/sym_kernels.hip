@@ -376,11 +376,11 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
         return good;
     }
     if constexpr (walk_mode<G>() == 2) {
-        // Pairs: both pages of the level in one loop (every level past the
-        // spiral's first few walks one or two pages; others take the rounds
-        // below).
+        // Pairs: both pages of the level in one loop.  Every cell i >= 1 of
+        // a spiral map walks one or two pages (a DFS level is at least 1;
+        // the tables are checked when they are staged).
         const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[i];
-        if ((lw & kLevelGeneric) == 0u) {
+        {
             const uint32_t a1 = lw & 0x7fu, a2 = (lw >> 8) & 0x7fu;
             const uint32_t r1 = G::kPageOff + ((lw >> 16) & 0xffu), r2 = G::kPageOff + (lw >> 24);
             uint32_t j1 = cyc_ld<G>(lds, cl, a1), j2 = cyc_ld<G>(lds, cl, a2);
@@ -437,7 +437,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             return and_and_not(reach1, reach2, above);
         }
     }
-    if constexpr (walk_mode<G>() >= 1) {
+    if constexpr (walk_mode<G>() == 1) {
         while (act != 0 && good != 0) {
             const uint32_t b = static_cast<uint32_t>(__builtin_ctz(act));
             act &= act - 1u;
@@ -663,7 +663,11 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
             flagged |= (((w >> (8u * p)) & 0xffu) < tid ? 0x80u : 0u) << (8u * p);
         }
         reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = flagged;
-        reinterpret_cast<uint32_t *>(lds + G::kLevelOff)[tid] = level_word<G>(w, tid);
+        const uint32_t lw = level_word<G>(w, tid);
+        reinterpret_cast<uint32_t *>(lds + G::kLevelOff)[tid] = lw;
+        // The paired walk takes every level of the map from this word.
+        if (walk_mode<G>() == 2 && tid >= 1 && tid <= a.last && (lw & kLevelGeneric) != 0u)
+            atomicOr(a.acc + ACC_ERROR, 4ull);
     }
     __syncthreads();
 }
