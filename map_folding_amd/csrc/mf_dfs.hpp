@@ -165,6 +165,7 @@ struct WideStack {
     static __device__ __forceinline__ uint32_t more(uint64_t e) { return (e & (e - 1)) != 0 ? 1u : 0u; }
     static __device__ __forceinline__ bool has_next(uint64_t e) { return (e & (e - 1)) != 0; }
     static __device__ __forceinline__ bool recompute(uint64_t) { return false; }
+    static __device__ __forceinline__ bool any_more(uint64_t e) { return (e & (e - 1)) != 0; }
     static __device__ __forceinline__ uint64_t advance(uint64_t e) { return e & (e - 1); }
     // The siblings after the current child as a mask (for splitting); never
     // unknown for this encoding.
@@ -233,6 +234,8 @@ struct CompactStackT {
     static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & kField; }
     // A recorded next sibling: advance() gives it.
     static __device__ __forceinline__ bool has_next(uint32_t e) { return e >= kNext; }
+    // Children after j, recorded or not.
+    static __device__ __forceinline__ bool any_more(uint32_t e) { return e >= kMore; }
     // No recorded sibling but more children: recompute the node above j.
     static __device__ __forceinline__ bool recompute(uint32_t e) { return e >= kMore && e < kNext; }
     // 0 = none, 1 = a recorded next sibling (advance()), 2 = recompute.

@@ -486,6 +486,12 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
     const uint32_t i0 = w.i;
     const uint32_t cb = cyc_addr<G>(ln.cl - P * G::kRowStride, i0);
     const uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + i0 * Stack::kSlotBytes;
+    // The child this step enters -- the node's first child, or after a
+    // backtrack the next sibling of the level it stops at -- goes in by one
+    // shared block at the end: its level's new stack entry and its cell.
+    bool enter = false, pop = w.popping;
+    typename Stack::Entry ent = 0;
+    uint32_t kid = 0;
     if (!w.popping) {
         uint32_t nw = 0, ns = 0;
         // A recompute (floor j + 1) keeps the children above j.
@@ -504,20 +510,19 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         // recomputed: its nodes enter no children).
         const bool leaf = i0 == last;
         if (leaf) w.leaves += cnt;
-        if (g != 0 && !leaf) {
-            const uint32_t j = mask_ctz(g);
-            Stack::put_at(lds, sb + P * Stack::kSlotBytes, Stack::encode_at(j, g, cnt));
-            place_at<G>(lds, ln, j, i0, cb + P * G::kRowStride);
-            w.i = i0 + 1;
-            return false;
+        enter = g != 0 && !leaf;
+        pop = !enter;
+        if (enter) {
+            kid = mask_ctz(g);
+            ent = Stack::encode_at(kid, g, cnt);
         }
     }
     // Backtracking, one level per pass while the lane pops: remove the level's
-    // child; the first level with a recorded sibling (entered after the passes)
-    // or with unrecorded ones (a recompute) ends it, and so does the item's
-    // root.  Each pass is one region with selects inside: a lane at its root
-    // rewrites its own cycle entry unchanged instead of branching.
-    bool pop = true, fin = false, adv = false;
+    // child.  A level with more children (recorded or not) stops it, and so
+    // does the item's root.  Each pass is one region with selects inside: a
+    // lane at its root rewrites its own cycle entry unchanged instead of
+    // branching.
+    bool fin = false, got = false;
     typename Stack::Entry ae = 0;
 #pragma unroll
     for (uint32_t k = 0; k < P; ++k) {
@@ -527,23 +532,26 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             const typename Stack::Entry e = Stack::get_at(lds, slot);
             const uint32_t c = cyc_ld_at<G>(lds, cell);
             const bool root = w.i == w.root;
-            const uint32_t j = Stack::child(e);
-            cyc_st<G>(lds, ln.cl, root ? w.i - 1 : j, c);   // unplace (at the root: unchanged)
-            const bool next = !root && Stack::has_next(e);
-            const bool rec = !root && Stack::recompute(e);
-            ae = next ? e : ae;
-            w.floor = rec ? j + 1 : w.floor;
+            cyc_st<G>(lds, ln.cl, root ? w.i - 1 : Stack::child(e), c);   // unplace (at the root: unchanged)
+            const bool more = !root && Stack::any_more(e);
+            ae = more ? e : ae;
+            got = more;
             w.i = root ? w.i : w.i - 1;
             fin = root;
-            adv = next;
-            pop = !(root || next || rec);
+            pop = !(root || more);
         }
     }
-    if (adv) {
-        // enter the recorded next sibling of level w.i
-        const typename Stack::Entry e2 = Stack::advance(ae);
-        Stack::put(lds, ln, w.i, e2);
-        place<G>(lds, ln, Stack::child(e2), w.i);
+    // Stopped at a level with more children: its recorded next sibling, or a
+    // recompute of the node above the current child.
+    const bool nx = got && Stack::has_next(ae);
+    if (got && !nx) w.floor = Stack::child(ae) + 1;
+    if (nx) {
+        ent = Stack::advance(ae);
+        kid = Stack::child(ent);
+    }
+    if (enter || nx) {
+        Stack::put(lds, ln, w.i, ent);
+        place<G>(lds, ln, kid, w.i);
         ++w.i;
     }
     w.popping = pop;
