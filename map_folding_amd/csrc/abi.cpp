@@ -32,6 +32,7 @@ uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
+int sym_walk_core(int core, const SymArgs &a);
 hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s);
 }  // namespace mf
 
@@ -592,12 +593,14 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
                     static_cast<unsigned long long>(opt->order_len), static_cast<unsigned long long>(f->num));
     if (f->kind == 1 || f->kind == 3) {
         const bool c96 = f->kind == 3;
-        const int core = c96 ? 2 : t.core;
-        if (core == 3 && t.schedule != MF_SCHED_RESPLIT)
+        if (!c96 && t.core == 3 && t.schedule != MF_SCHED_RESPLIT)
             return fail(MF_ERR_ARG, "core 3 (two lanes per subtree) runs the re-split schedule only");
         SymArgs a;
         uint32_t levels = 0;
         sym_args(*f, a, &levels);
+        // Numberings whose levels do not all walk one or two pages run one
+        // page walk at a time.
+        const int core = c96 ? 2 : sym_walk_core(t.core, a);
         const uint32_t cells = static_cast<uint32_t>(f->sym.n2);
         const uint32_t lds = sym_lds_bytes(core, levels, t.block, cells);
         auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
@@ -640,7 +643,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
                 uint64_t deep_num = 0;
                 double deep_ms = 0.0;
                 // The deepening pass runs one lane per item (the compact-stack instance).
-                const int dcore = core == 3 ? 2 : core;
+                const int dcore = core == 3 ? sym_walk_core(2, a) : core;
                 rc = deepen_frontier(a, dcore, t, cells, dev, f->num, origins, per_count != nullptr,
                                      per_nodes != nullptr, static_cast<uint32_t>(target),
                                      per_item ? item_targets.data() : nullptr, &deep, &deep_num, &deep_ms);

@@ -38,6 +38,7 @@ hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid,
                       bool events, uint32_t cells, hipStream_t);
 uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells);
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells);
+int sym_walk_core(int core, const SymArgs &a);
 int lib_error(int status, const char *msg);
 bool valid_case_cycle(const uint8_t *cycle, int level);
 WorkItem make_sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin);
@@ -143,7 +144,7 @@ extern "C" int mf_sym_solve_noalloc(int32_t n, int32_t level, const uint8_t *cyc
     }
 
     // Launch configuration: the mf_sym_solve defaults.
-    const int core = c96 ? 2 : ((opt && opt->core == 1) ? 1 : 2);
+    int core = c96 ? 2 : ((opt && opt->core == 1) ? 1 : 2);
     uint32_t block = (opt && opt->block > 0) ? static_cast<uint32_t>(opt->block) : 256u;
     block = std::max(64u, std::min(256u, (block / 64u) * 64u));
     const uint32_t waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
@@ -160,6 +161,7 @@ extern "C" int mf_sym_solve_noalloc(int32_t n, int32_t level, const uint8_t *cyc
     a.levels = std::max<uint32_t>(1u, a.last - a.base_level);
     a.budget = budget;
     a.tail = tail;
+    if (!c96) core = sym_walk_core(core, a);
     const uint32_t cells = static_cast<uint32_t>(pr.n2);
     const uint32_t lds = sym_lds_bytes(core, a.levels, block, cells);
     const void *kp = sym_kernel_ptr(core, true, events, cells);

@@ -65,9 +65,12 @@ namespace {
 #define MF_SYM_ROW_STRIDE 68
 #endif
 template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_,
-          uint32_t kRowStride_ = MF_SYM_ROW_STRIDE, uint32_t kLanes_ = 1>
+          uint32_t kRowStride_ = MF_SYM_ROW_STRIDE, uint32_t kLanes_ = 1, int kWalk_ = -1>
 struct Geometry {
     static constexpr uint32_t kCells = kCells_;
+    // get_good's walks: 1 one page at a time, 2 a level's one or two pages
+    // in one paired loop, -1 by mask width (walk_mode below).
+    static constexpr int kWalk = kWalk_;
     // Lanes per subtree: 1, or 2 (a lane pair shares one row and one stack,
     // keeps identical DFS state, and splits each node's page walks).
     static constexpr uint32_t kLanes = kLanes_;
@@ -108,14 +111,20 @@ using G49 = Geometry<64, uint64_t, WorkItem, CompactStack, 49>;
 // Two lanes per subtree, up to 8x8: 32 rows per wave at a 36-byte stride (9
 // dwords, odd) and 32 stacks, half the LDS of G64 per wave.
 using G64Pair = Geometry<64, uint64_t, WorkItem, CompactStackT<uint64_t, 6, 32>, 64, 36, 2>;
+// Up to 8x8 with one page walk at a time: for cell numberings in which some
+// level walks no page or more than two (ring orders, other centers), which
+// the paired walk does not cover.
+using G64Any = Geometry<64, uint64_t, WorkItem, CompactStack, 64, MF_SYM_ROW_STRIDE, 1, 1>;
 
 // The geometry of a launch: `cells` = n*n of the map; core 3 = two lanes per
-// subtree (re-split schedule, up to 8x8).
-enum class Geo { k49, k64, k64Wide, k96, k64Pair };
+// subtree (re-split schedule, up to 8x8); core 4 = one page walk at a time
+// (sym_walk_core picks it).
+enum class Geo { k49, k64, k64Wide, k96, k64Pair, k64Any };
 inline Geo geometry(int core, uint32_t cells) {
     if (cells > 64) return Geo::k96;
     if (core == 1) return Geo::k64Wide;
     if (core == 3) return Geo::k64Pair;
+    if (core == 4) return Geo::k64Any;
     return cells <= 49 ? Geo::k49 : Geo::k64;
 }
 
@@ -231,7 +240,11 @@ struct Totals {
 // the 8x8 and 7x7 chunks than every step; three per trip is slower again).
 template <typename G>
 constexpr int walk_mode() {
-    return MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE : (sizeof(typename G::Mask) > 8 ? 1 : 2);
+    // (Two lanes per subtree split a level's pages between the pair.)
+    return G::kLanes == 2         ? 1
+           : MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE
+           : G::kWalk >= 0         ? G::kWalk
+                                   : (sizeof(typename G::Mask) > 8 ? 1 : 2);
 }
 
 // Where a page-p walk moves from cell j at level i: the page partner if it is
@@ -304,11 +317,11 @@ __device__ __forceinline__ uint64_t or3(uint64_t a, uint64_t b, uint64_t c) {
 // walked page's anchor (its partner of cell i), bit 7 set when a second page
 // is walked, bits 8-14 the second anchor, bits 16-23 / 24-31 the two pages'
 // partner-row offsets from kPageOff.  With one page the second repeats the
-// first.  kLevelGeneric: no page, more than two, or rows past a byte (9x9):
-// the walk takes the page-by-page path.
+// first.  kLevelGeneric: no page, more than two, or rows past a byte (9x9);
+// a map with such a level runs one page walk at a time (sym_walk_core).
 constexpr uint32_t kLevelGeneric = 0x8000u;
 template <typename G>
-__device__ __forceinline__ uint32_t level_word(uint32_t partners, uint32_t i) {
+__host__ __device__ __forceinline__ uint32_t level_word(uint32_t partners, uint32_t i) {
     uint32_t n = 0, a0 = 0, a1 = 0, r0 = 0, r1 = 0;
 #pragma unroll
     for (uint32_t p = 0; p < 4; ++p) {
@@ -377,8 +390,9 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
     }
     if constexpr (walk_mode<G>() == 2) {
         // Pairs: both pages of the level in one loop.  Every cell i >= 1 of
-        // a spiral map walks one or two pages (a DFS level is at least 1;
-        // the tables are checked when they are staged).
+        // a spiral map walks one or two pages (a DFS level is at least 1);
+        // other maps run one page at a time, and a kernel staged with any
+        // other level stops before its first walk.
         const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[i];
         {
             const uint32_t a1 = lw & 0x7fu, a2 = (lw >> 8) & 0x7fu;
@@ -655,12 +669,11 @@ __device__ __forceinline__ Lane make_lane(uint32_t levels, uint32_t base_level) 
     return ln;
 }
 
+// Returns false (for the whole block, with ACC_ERROR bit 4 raised) when the
+// tables do not fit this geometry: the kernel then returns without walking.
 template <typename G>
-__device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
+__device__ __forceinline__ bool stage_tables(const SymArgs &a, uint8_t *lds) {
     const uint32_t tid = threadIdx.x;
-    // The walk addresses the page table by absolute LDS offset.
-    const auto *local = (const __attribute__((address_space(3))) uint8_t *)lds;
-    if (tid == 0 && static_cast<uint32_t>(reinterpret_cast<size_t>(local)) != 0u) atomicOr(a.acc + ACC_ERROR, 4ull);
     if (tid < G::kCells) {
         // Partners are at most n2 <= 81 < 128: bit 7 of each byte is free.
         const uint32_t w = a.edge_word[tid];
@@ -671,13 +684,25 @@ __device__ __forceinline__ void stage_tables(const SymArgs &a, uint8_t *lds) {
             flagged |= (((w >> (8u * p)) & 0xffu) < tid ? 0x80u : 0u) << (8u * p);
         }
         reinterpret_cast<uint32_t *>(lds + G::kPartnerOff)[tid] = flagged;
-        const uint32_t lw = level_word<G>(w, tid);
-        reinterpret_cast<uint32_t *>(lds + G::kLevelOff)[tid] = lw;
-        // The paired walk takes every level of the map from this word.
-        if (walk_mode<G>() == 2 && tid >= 1 && tid <= a.last && (lw & kLevelGeneric) != 0u)
-            atomicOr(a.acc + ACC_ERROR, 4ull);
+        reinterpret_cast<uint32_t *>(lds + G::kLevelOff)[tid] = level_word<G>(w, tid);
     }
     __syncthreads();
+    // Checked by every wave on its own (a block-wide reduction would take
+    // static LDS and move the dynamic block).  The walk addresses the page
+    // table by absolute LDS offset.
+    const auto *local = (const __attribute__((address_space(3))) uint8_t *)lds;
+    bool bad = static_cast<uint32_t>(reinterpret_cast<size_t>(local)) != 0u;
+    if constexpr (walk_mode<G>() == 2) {
+        // The paired walk takes every level of the map from its level word:
+        // a level that walks no page or more than two would send it round an
+        // orbit that never returns to its anchor.  (Levels 1..last of at most
+        // 64 cells, one per lane; wider tables have no paired level at all.)
+        const uint32_t c = tid & 63u;
+        const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[c];
+        bad = bad || __any(c >= 1u && c <= a.last && (lw & kLevelGeneric) != 0u);
+    }
+    if (bad && tid == 0) atomicOr(a.acc + ACC_ERROR, 4ull);
+    return !bad;
 }
 
 // Split everything the lane holds into the spill queue: the node about to be
@@ -756,7 +781,7 @@ __device__ __forceinline__ bool spill_all(const SymArgs &a, uint8_t *lds, const 
 template <typename G, bool EV>
 __global__ __launch_bounds__(256) void sym_static_kernel(SymArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    stage_tables<G>(a, lds);
+    if (!stage_tables<G>(a, lds)) return;
     const Lane ln = make_lane<G>(a.levels, a.base_level);
     const auto *items = static_cast<const typename G::Item *>(a.items);
     static_assert(G::kLanes == 1, "the static schedule runs one lane per item");
@@ -794,7 +819,7 @@ __device__ __forceinline__ void persistent_body(const SymArgs &a) {
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.acc + ACC_ROUNDS, 1ull);
-    stage_tables<G>(a, lds);
+    if (!stage_tables<G>(a, lds)) return;
     const uint32_t lane = threadIdx.x & 63u;
     const Lane ln = make_lane<G>(a.levels, a.base_level);
     const auto *items = static_cast<const typename G::Item *>(a.items);
@@ -889,7 +914,7 @@ __global__ __launch_bounds__(256) void sym_deepen_kernel(SymArgs a) {
     using Mask = typename G::Mask;
     using Item = typename G::Item;
     extern __shared__ __align__(16) uint8_t lds[];
-    stage_tables<G>(a, lds);
+    if (!stage_tables<G>(a, lds)) return;
     const uint32_t lane = threadIdx.x & 63u;
     const Lane ln = make_lane<G>(1, 0);
     const auto *items = static_cast<const Item *>(a.items);
@@ -985,7 +1010,7 @@ __global__ __launch_bounds__(256) void sym_features_kernel(SymArgs a, float *out
     using Mask = typename G::Mask;
     using Item = typename G::Item;
     extern __shared__ __align__(16) uint8_t lds[];
-    stage_tables<G>(a, lds);
+    if (!stage_tables<G>(a, lds)) return;
     const uint32_t lane = threadIdx.x & 63u;
     const Lane ln = make_lane<G>(kFeatureStackLevels, 0);
     const uint32_t sbase = ln.sl - lane * G::Stack::kLaneBytes;   // this wave's stack region
@@ -1077,14 +1102,33 @@ __global__ __launch_bounds__(256) void sym_features_kernel(SymArgs a, float *out
         f[72 + q] = static_cast<float>(log2(1.0 + static_cast<double>(prof[q]))) / 24.0f;
 }
 
-hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s) {
-    // Rows of 64 cells for every square up to 8x8.
-    const uint32_t lds = G64::kTableBytes + 4u * (G64::kCycleBytes + kFeatureStackLevels * G64::Stack::kSlotBytes);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sym_features_kernel<G64>),
+namespace {
+template <typename G>
+hipError_t launch_features(const SymArgs &a, float *out, hipStream_t s) {
+    const uint32_t lds = G::kTableBytes + 4u * (G::kCycleBytes + kFeatureStackLevels * G::Stack::kSlotBytes);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&sym_features_kernel<G>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(sym_features_kernel<G64>, dim3((a.num_items + 255u) / 256u), dim3(256), lds, s, a, out);
+    hipLaunchKernelGGL(sym_features_kernel<G>, dim3((a.num_items + 255u) / 256u), dim3(256), lds, s, a, out);
     return hipGetLastError();
+}
+}  // namespace
+
+// The core that runs `core`'s request for the map in `a`: the u64 paired
+// walk (cores 0-2) needs every level 1..last to walk one or two pages, which
+// every spiral numbering gives; a map with any other level runs core 4 (one
+// page walk at a time, compact stack).  Core 3's two-lane walk and the 9x9
+// geometry already walk page by page.
+int sym_walk_core(int core, const SymArgs &a) {
+    if (core == 3 || core == 4) return core;
+    for (uint32_t i = 1; i <= a.last && i < 64u; ++i)
+        if ((level_word<G64>(a.edge_word[i], i) & kLevelGeneric) != 0u) return 4;
+    return core;
+}
+
+hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s) {
+    // Rows of 64 cells for every square up to 8x8.
+    return sym_walk_core(0, a) == 4 ? launch_features<G64Any>(a, out, s) : launch_features<G64>(a, out, s);
 }
 
 // Host-visible entry points (C++ linkage inside the library).
@@ -1098,7 +1142,8 @@ const void *pick(bool persistent, bool events) {
 }
 }  // namespace
 
-// core: 1 = wide u64 stack (up to 8x8), otherwise the compact 16-bit stack;
+// core: 1 = wide u64 stack (up to 8x8), 3 = two lanes per subtree, 4 = one
+// page walk at a time (sym_walk_core), otherwise the compact 16-bit stack;
 // cells = n*n picks the row size (49, 64 or, for 9x9, 81 entries).
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells) {
     switch (geometry(core, cells)) {
@@ -1108,6 +1153,7 @@ const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cell
                           : reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, false>);
         case Geo::k96: return pick<G96>(persistent, events);
         case Geo::k64Wide: return pick<G64Wide>(persistent, events);
+        case Geo::k64Any: return pick<G64Any>(persistent, events);
         case Geo::k49:
             if (persistent)
                 return events ? reinterpret_cast<const void *>(&sym_persistent_kernel49<true>)
@@ -1152,6 +1198,8 @@ const void *sym_deepen_ptr(int core, bool events, uint32_t cells) {
                                      : reinterpret_cast<const void *>(&sym_deepen_kernel<G49, false>);
         case Geo::k64: return events ? reinterpret_cast<const void *>(&sym_deepen_kernel<G64, true>)
                                      : reinterpret_cast<const void *>(&sym_deepen_kernel<G64, false>);
+        case Geo::k64Any: return events ? reinterpret_cast<const void *>(&sym_deepen_kernel<G64Any, true>)
+                                        : reinterpret_cast<const void *>(&sym_deepen_kernel<G64Any, false>);
         default: return nullptr;
     }
 }
@@ -1161,6 +1209,7 @@ uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells
         case Geo::k64Pair: return lds_of<G64Pair>(levels, block);
         case Geo::k96: return lds_of<G96>(levels, block);
         case Geo::k64Wide: return lds_of<G64Wide>(levels, block);
+        case Geo::k64Any: return lds_of<G64Any>(levels, block);
         case Geo::k49: return lds_of<G49>(levels, block);
         default: return lds_of<G64>(levels, block);
     }

@@ -63,6 +63,29 @@ def test_sym9_per_case_and_events(gpu, oracle, sched, e, start, m):
         st["nodes"], st["page_walks"], st["walk_steps"], st["leaves"])
 
 
+@pytest.mark.parametrize("order", ["reverse", "corners_first"])
+def test_ring_orders_on_u64_cores(gpu, oracle, monkeypatch, order):
+    """7x7 under a ring order (sym_fold9.cpp:160-229): some levels walk no
+    page or more than two (4 under reverse, 16 under corners_first), which
+    the paired two-page walk of the u64 kernels does not cover; the library
+    runs such maps one page walk at a time.  Every schedule and core, and
+    the deepening pass, count the oracle's per-case totals and events."""
+    monkeypatch.setenv("MF_ORDER", order)   # the oracle reads the reference's environment
+    ocyc, omul = oracle.sym9_cases(7, 26, 3000, 300)
+    pc0, pn0, tot, st = oracle.sym9_solve(7, 26, ocyc, omul)
+    with gpu.sym_numbering(order=order):
+        cyc, mul = gpu.sym_cases(7, 26, 3000, 300)
+        assert np.array_equal(cyc, ocyc[:, :cyc.shape[1]]) and np.array_equal(mul, omul)
+        for opts in (dict(schedule=0), dict(schedule=1), dict(schedule=1, deepen=30),
+                     dict(schedule=1, core=1), dict(schedule=1, core=3)):
+            _say(f"sym 7x7 {order} {opts}")
+            res, pc, pn = gpu.sym_solve(7, 26, cyc, mul, gpu.SolveOptions(**opts), per_item=True)
+            assert np.array_equal(pc, pc0) and np.array_equal(pn, pn0)
+            assert res.count == tot
+            assert (res.nodes, res.walks, res.steps, res.leaves) == (
+                st["nodes"], st["page_walks"], st["walk_steps"], st["leaves"])
+
+
 def test_sym9_seed_slice_windows(gpu):
     """Seed slices (MF_SEED_START / MF_SEED_COUNT): windows are additive, and
     a window that crosses from one seed into the next is the two seeds'
