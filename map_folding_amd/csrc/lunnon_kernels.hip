@@ -118,6 +118,21 @@ constexpr uint32_t kLMaxPops = MF_LUNNON_MAX_POPS;
 
 template <typename Stack, bool EV>
 __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w) {
+    constexpr uint32_t P = kLMaxPops;
+    // The link entry and stack slot of leaf l - P: those of leaves l - P .. l
+    // sit at constant offsets from them, which fold into the LDS instructions.
+    // (The bases may point below the lane's region; a lane at its root reads
+    // the entries of leaf root - 1, inside the wave's rows, and rewrites its
+    // link entry unchanged.)
+    const uint32_t l0 = w.l;
+    const uint32_t rb = row_at<kLStride>(ln.cl - P * kLStride, l0);
+    const uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + l0 * Stack::kSlotBytes;
+    // The child this step enters -- the node's first position, or after a
+    // backtrack the next recorded sibling -- goes in by one shared block at
+    // the end: its level's stack entry and its link.
+    bool enter = false, pop = w.popping;
+    typename Stack::Entry ent = 0;
+    uint32_t kid = 0;
     if (!w.popping) {
         uint32_t ns = 0, nc = 0;
         uint64_t g = gap_mask(lds, ln, w, ns, nc);
@@ -129,48 +144,61 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
                 w.isteps += ns;
                 w.icands += nc;
             }
-            g &= ~1ull;   // position 0: pruned by the normal-form filter (l >= 2)
-            if (w.l == w.n) {
-                w.comps += static_cast<uint32_t>(__popcll(g));
-                g = 0;
-            }
-        } else {
-            g &= ~below_mask(w.floor);
-            w.floor = 0;
         }
-        if (g != 0) {
-            const typename Stack::Entry e = Stack::encode(g);
-            Stack::put(lds, ln, w.l, e);
-            link(lds, ln, Stack::child(e), w.l);
-            ++w.l;
-            return false;
-        }
-    }
-    for (uint32_t k = 0; k < kLMaxPops; ++k) {
-        if (w.l == w.root) {
-            w.popping = false;
-            return true;
-        }
-        --w.l;
-        const typename Stack::Entry e = Stack::get(lds, ln, w.l);
-        const uint32_t j = Stack::child(e);
-        unlink(lds, ln, j, w.l);
-        if (Stack::has_next(e)) {
-            const typename Stack::Entry e2 = Stack::advance(e);
-            Stack::put(lds, ln, w.l, e2);
-            link(lds, ln, Stack::child(e2), w.l);
-            ++w.l;
-            w.popping = false;
-            return false;
-        }
-        if (Stack::recompute(e)) {
-            w.floor = j + 1;
-            w.popping = false;
-            return false;
+        // Position 0 is pruned by the normal-form filter (l >= 2); a
+        // recompute keeps the positions above its floor j + 1 >= 1.
+        g &= ~0ull << (w.floor == 0 ? 1u : w.floor);
+        w.floor = 0;
+        // Leaf n's positions are completions (a recompute never reaches n:
+        // its inserts enter no children).
+        const bool leaf = l0 == w.n;
+        if (leaf) w.comps += static_cast<uint32_t>(__popcll(g));
+        enter = g != 0 && !leaf;
+        pop = !enter;
+        if (enter) {
+            kid = mask_ctz(g);
+            ent = Stack::encode_at(kid, g, mask_popcount(g));
         }
     }
-    w.popping = true;
-    return false;
+    // Backtracking, one leaf per pass while the lane pops: unlink it from
+    // below its position.  A level with more positions (recorded or not)
+    // stops it, and so does the item's root.  Each pass is one region with
+    // selects inside.
+    bool fin = false, got = false;
+    typename Stack::Entry ae = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < P; ++k) {
+        if (pop) {
+            // leaf l0 - 1 - k
+            const uint32_t slot = sb + (P - 1 - k) * Stack::kSlotBytes, at = rb + (P - 1 - k) * kLStride;
+            const typename Stack::Entry e = Stack::get_at(lds, slot);
+            const uint32_t c = row_ld<kLStride>(lds, at);
+            const bool root = w.l == w.root;
+            // unlink (mf.c:131-132): b[j] = b[l]; at the root b[l] unchanged
+            row_st<kLStride>(lds, root ? at : row_at<kLStride>(ln.cl, Stack::child(e)), c);
+            const bool more = !root && Stack::any_more(e);
+            ae = more ? e : ae;
+            got = more;
+            w.l = root ? w.l : w.l - 1;
+            fin = root;
+            pop = !(root || more);
+        }
+    }
+    // Stopped at a level with more positions: its recorded next sibling, or
+    // a recompute of the leaf's gaps above the current position.
+    const bool nx = got && Stack::has_next(ae);
+    if (got && !nx) w.floor = Stack::child(ae) + 1;
+    if (nx) {
+        ent = Stack::advance(ae);
+        kid = Stack::child(ent);
+    }
+    if (enter || nx) {
+        Stack::put(lds, ln, w.l, ent);
+        link(lds, ln, kid, w.l);
+        ++w.l;
+    }
+    w.popping = pop;
+    return fin;
 }
 
 // Item kinds as for the symmetry DFS; the floor of a recompute piece travels
