@@ -64,6 +64,36 @@ namespace {
 #ifndef MF_SYM_ROW_STRIDE
 #define MF_SYM_ROW_STRIDE 68
 #endif
+// Cells below i / at or above f as masks.  For u128 masks (i, f < 128) the
+// generic 128-bit shifts cost about a dozen instructions; here one
+// v_lshlrev_b64 of all ones (it shifts by the amount's low 6 bits), bit 6 of
+// the amount as an all-ones / zero mask m, and one v_bitop3_b32 per dword
+// (truth table of (S0, S1) = (0xF0, 0xCC), S2 unused): with s = ~0 << (i & 63),
+// below(i) = (~s | m, ~s & m) -> 0xCF, 0x0C; from(f) = (s & ~m, s | ~m) ->
+// 0x30, 0xF3 (low half, high half).
+__device__ __forceinline__ uint64_t mask_below(uint64_t, uint32_t i) { return (1ull << i) - 1ull; }
+__device__ __forceinline__ uint64_t mask_from(uint64_t, uint32_t f) { return ~0ull << f; }
+template <uint32_t kLo, uint32_t kHi>
+__device__ __forceinline__ unsigned __int128 halves128(uint32_t i) {
+    uint64_t sh;
+    asm("v_lshlrev_b64 %0, %1, -1" : "=v"(sh) : "v"(i));
+    const uint32_t m = static_cast<uint32_t>(static_cast<int32_t>(i << 25) >> 31);   // i >= 64
+    const uint32_t s0 = static_cast<uint32_t>(sh), s1 = static_cast<uint32_t>(sh >> 32);
+    uint32_t d0, d1, d2, d3;
+    asm("v_bitop3_b32 %0, %1, %2, %2 bitop3:%3" : "=v"(d0) : "v"(s0), "v"(m), "i"(kLo));
+    asm("v_bitop3_b32 %0, %1, %2, %2 bitop3:%3" : "=v"(d1) : "v"(s1), "v"(m), "i"(kLo));
+    asm("v_bitop3_b32 %0, %1, %2, %2 bitop3:%3" : "=v"(d2) : "v"(s0), "v"(m), "i"(kHi));
+    asm("v_bitop3_b32 %0, %1, %2, %2 bitop3:%3" : "=v"(d3) : "v"(s1), "v"(m), "i"(kHi));
+    return (static_cast<unsigned __int128>((static_cast<uint64_t>(d3) << 32) | d2) << 64) |
+           ((static_cast<uint64_t>(d1) << 32) | d0);
+}
+[[maybe_unused]] __device__ __forceinline__ unsigned __int128 mask_below(unsigned __int128, uint32_t i) {
+    return halves128<0xCF, 0x0C>(i);
+}
+[[maybe_unused]] __device__ __forceinline__ unsigned __int128 mask_from(unsigned __int128, uint32_t f) {
+    return halves128<0x30, 0xF3>(f);
+}
+
 template <uint32_t kCells_, typename Mask_, typename Item_, typename Stack_, uint32_t kRowCells_ = kCells_,
           uint32_t kRowStride_ = MF_SYM_ROW_STRIDE, uint32_t kLanes_ = 1, int kWalk_ = -1>
 struct Geometry {
@@ -88,7 +118,9 @@ struct Geometry {
     static constexpr uint32_t kStateBytes = kCells;
     // Cells below i.  i is a level (at most n2 - 1) or a floor j + 1 with
     // j < i, so it stays under the mask width (64 up to 8x8, 128 for 9x9).
-    static __device__ __forceinline__ Mask below(uint32_t i) { return (static_cast<Mask>(1) << i) - 1; }
+    static __device__ __forceinline__ Mask below(uint32_t i) { return mask_below(static_cast<Mask>(0), i); }
+    // Cells at or above f (a recompute's floor, f < the mask width).
+    static __device__ __forceinline__ Mask from(uint32_t f) { return mask_from(static_cast<Mask>(0), f); }
     static __device__ __forceinline__ Mask item_mask(const WorkItem *it) { return it->mask; }
     static __device__ __forceinline__ Mask item_mask(const WorkItem128 *it) {
         return (static_cast<Mask>(it->mask_hi) << 64) | it->mask_lo;
@@ -259,18 +291,41 @@ __device__ __forceinline__ uint32_t hop(uint32_t prow, uint32_t i, uint32_t j) {
     return v < i ? v : j;
 }
 
+// reach |= bit j.  For u128 masks (cells < 128) the generic 128-bit shift
+// costs about 15 instructions; here: the bit in a 64-bit word
+// (v_lshlrev_b64 shifts by the amount's low 6 bits), bit 6 of j as an
+// all-ones / zero mask, and one v_bitop3_b32 per dword that ORs in the word
+// under its half's mask (truth table of (S0, S1, S2) = (0xF0, 0xCC, 0xAA):
+// S0 | (S1 & ~S2) = 0xF4, S0 | (S1 & S2) = 0xF8).
+__device__ __forceinline__ void or_bit(uint64_t &r, uint32_t j) { r |= 1ull << j; }
+[[maybe_unused]] __device__ __forceinline__ void or_bit(unsigned __int128 &r, uint32_t j) {
+    uint64_t b;
+    asm("v_lshlrev_b64 %0, %1, 1" : "=v"(b) : "v"(j));
+    const uint32_t hi = static_cast<uint32_t>(static_cast<int32_t>(j << 25) >> 31);   // j >= 64
+    const uint32_t bl = static_cast<uint32_t>(b), bh = static_cast<uint32_t>(b >> 32);
+    uint32_t d0 = static_cast<uint32_t>(r), d1 = static_cast<uint32_t>(r >> 32), d2 = static_cast<uint32_t>(r >> 64),
+             d3 = static_cast<uint32_t>(r >> 96);
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xf4" : "=v"(d0) : "v"(d0), "v"(bl), "v"(hi));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xf4" : "=v"(d1) : "v"(d1), "v"(bh), "v"(hi));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xf8" : "=v"(d2) : "v"(d2), "v"(bl), "v"(hi));
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xf8" : "=v"(d3) : "v"(d3), "v"(bh), "v"(hi));
+    r = (static_cast<unsigned __int128>((static_cast<uint64_t>(d3) << 32) | d2) << 64) |
+        ((static_cast<uint64_t>(d1) << 32) | d0);
+}
+
 // One page walk from `anchor`: every cell reached is a legal insertion point.
 template <typename G>
 __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict__ lds, uint32_t cl, uint32_t i,
                                                       uint32_t prow, uint32_t anchor, uint32_t &steps) {
     using Mask = typename G::Mask;
-    Mask reach = static_cast<Mask>(1) << anchor;
+    Mask reach = 0;
+    or_bit(reach, anchor);
     uint32_t j = cyc_ld<G>(lds, cl, anchor);
     __builtin_amdgcn_s_setprio(1);   // as in the paired walk (good_mask)
     while (j != anchor) {
         ++steps;
         j = hop<G>(prow, i, j);
-        reach |= static_cast<Mask>(1) << j;
+        or_bit(reach, j);
         j = cyc_ld<G>(lds, cl, j);
     }
     __builtin_amdgcn_s_setprio(0);
@@ -510,7 +565,7 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         uint32_t nw = 0, ns = 0;
         // A recompute (floor j + 1) keeps the children above j.
         const typename G::Mask g =
-            good_mask<G, EV>(lds, ln.cl, i0, nw, ns) & (~static_cast<typename G::Mask>(0) << w.floor);
+            good_mask<G, EV>(lds, ln.cl, i0, nw, ns) & G::from(w.floor);
         const uint32_t cnt = mask_popcount(g);   // children of this node
         if (w.floor == 0) {
             ++w.inodes;
