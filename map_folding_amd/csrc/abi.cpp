@@ -32,7 +32,7 @@ uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
-int sym_walk_core(int core, const SymArgs &a);
+int sym_walk_core(int core, const SymArgs &a, uint32_t cells);
 hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s);
 }  // namespace mf
 
@@ -600,7 +600,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         sym_args(*f, a, &levels);
         // Numberings whose levels do not all walk one or two pages run one
         // page walk at a time.
-        const int core = c96 ? 2 : sym_walk_core(t.core, a);
+        const int core = sym_walk_core(c96 ? 2 : t.core, a, static_cast<uint32_t>(f->sym.n2));
         const uint32_t cells = static_cast<uint32_t>(f->sym.n2);
         const uint32_t lds = sym_lds_bytes(core, levels, t.block, cells);
         auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
@@ -643,7 +643,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
                 uint64_t deep_num = 0;
                 double deep_ms = 0.0;
                 // The deepening pass runs one lane per item (the compact-stack instance).
-                const int dcore = core == 3 ? sym_walk_core(2, a) : core;
+                const int dcore = core == 3 ? sym_walk_core(2, a, cells) : core;
                 rc = deepen_frontier(a, dcore, t, cells, dev, f->num, origins, per_count != nullptr,
                                      per_nodes != nullptr, static_cast<uint32_t>(target),
                                      per_item ? item_targets.data() : nullptr, &deep, &deep_num, &deep_ms);

@@ -38,7 +38,7 @@ hipError_t launch_sym(const SymArgs &, int core, bool persistent, uint32_t grid,
                       bool events, uint32_t cells, hipStream_t);
 uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells);
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells);
-int sym_walk_core(int core, const SymArgs &a);
+int sym_walk_core(int core, const SymArgs &a, uint32_t cells);
 int lib_error(int status, const char *msg);
 bool valid_case_cycle(const uint8_t *cycle, int level);
 WorkItem make_sym_item(const uint8_t *cycle, uint64_t mult, int level, uint32_t origin);
@@ -161,8 +161,8 @@ extern "C" int mf_sym_solve_noalloc(int32_t n, int32_t level, const uint8_t *cyc
     a.levels = std::max<uint32_t>(1u, a.last - a.base_level);
     a.budget = budget;
     a.tail = tail;
-    if (!c96) core = sym_walk_core(core, a);
     const uint32_t cells = static_cast<uint32_t>(pr.n2);
+    core = sym_walk_core(core, a, cells);
     const uint32_t lds = sym_lds_bytes(core, a.levels, block, cells);
     const void *kp = sym_kernel_ptr(core, true, events, cells);
     NA_TRY(hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));

@@ -147,13 +147,14 @@ using G64Pair = Geometry<64, uint64_t, WorkItem, CompactStackT<uint64_t, 6, 32>,
 // level walks no page or more than two (ring orders, other centers), which
 // the paired walk does not cover.
 using G64Any = Geometry<64, uint64_t, WorkItem, CompactStack, 64, MF_SYM_ROW_STRIDE, 1, 1>;
+using G96Any = Geometry<96, unsigned __int128, WorkItem128, CompactStack128, 81, 64, 1, 1>;
 
 // The geometry of a launch: `cells` = n*n of the map; core 3 = two lanes per
-// subtree (re-split schedule, up to 8x8); core 4 = one page walk at a time
-// (sym_walk_core picks it).
-enum class Geo { k49, k64, k64Wide, k96, k64Pair, k64Any };
+// subtree (re-split schedule, up to 8x8); cores 4 (up to 8x8) and 5 (9x9) =
+// one page walk at a time (sym_walk_core picks them).
+enum class Geo { k49, k64, k64Wide, k96, k64Pair, k64Any, k96Any };
 inline Geo geometry(int core, uint32_t cells) {
-    if (cells > 64) return Geo::k96;
+    if (cells > 64) return core == 5 ? Geo::k96Any : Geo::k96;
     if (core == 1) return Geo::k64Wide;
     if (core == 3) return Geo::k64Pair;
     if (core == 4) return Geo::k64Any;
@@ -260,10 +261,12 @@ struct Totals {
 //      chain walks its orbit again under a scalar open mask, which adds
 //      nothing to its reach: 7 VALU per chain step.  13% less time than
 //      mode 1 on the 8x8 chunk.
-// The default is pairs for u64 masks and rounds for u128 masks (9x9), where
-// two 128-bit bit-sets per iteration cost more than the overlap saves (5.6%
-// slower than rounds; 4% with closed chains unmasked).  MF_SYM_WALK_MODE
-// overrides it.
+// The default is pairs for every one-lane geometry whose levels all walk one
+// or two pages (every spiral numbering; G64Any and G96Any, for the other
+// numberings, run rounds).  For u128 masks (9x9) pairs pay since each reach
+// bit is 6 VALU (or_bit): 10% less time than rounds on the 9x9 depth-44
+// window (with the generic 128-bit shifts they had been 5.6% slower).
+// MF_SYM_WALK_MODE overrides it.
 #ifndef MF_SYM_WALK_MODE
 #define MF_SYM_WALK_MODE -1
 #endif
@@ -276,7 +279,7 @@ constexpr int walk_mode() {
     return G::kLanes == 2         ? 1
            : MF_SYM_WALK_MODE >= 0 ? MF_SYM_WALK_MODE
            : G::kWalk >= 0         ? G::kWalk
-                                   : (sizeof(typename G::Mask) > 8 ? 1 : 2);
+                                   : 2;
 }
 
 // Where a page-p walk moves from cell j at level i: the page partner if it is
@@ -365,18 +368,29 @@ __device__ __forceinline__ uint64_t or3(uint64_t a, uint64_t b, uint64_t c) {
 }
 [[maybe_unused]] __device__ __forceinline__ unsigned __int128 and_and_not(unsigned __int128 a, unsigned __int128 b,
                                                          unsigned __int128 m) {
-    return a & b & ~m;
+    const uint64_t lo = and_and_not(static_cast<uint64_t>(a), static_cast<uint64_t>(b), static_cast<uint64_t>(m));
+    const uint64_t hi = and_and_not(static_cast<uint64_t>(a >> 64), static_cast<uint64_t>(b >> 64),
+                                    static_cast<uint64_t>(m >> 64));
+    return (static_cast<unsigned __int128>(hi) << 64) | lo;
 }
 
 // The pages get_good walks at level i, as one word: bits 0-6 the first
 // walked page's anchor (its partner of cell i), bit 7 set when a second page
 // is walked, bits 8-14 the second anchor, bits 16-23 / 24-31 the two pages'
-// partner-row offsets from kPageOff.  With one page the second repeats the
-// first.  kLevelGeneric: no page, more than two, or rows past a byte (9x9);
-// a map with such a level runs one page walk at a time (sym_walk_core).
+// partner-row offsets from kPageOff, shifted right by row_shift (96-cell
+// tables: the offsets 96p are multiples of 32).  With one page the second
+// repeats the first.  kLevelGeneric: no page or more than two; a map with
+// such a level runs one page walk at a time (sym_walk_core).
 constexpr uint32_t kLevelGeneric = 0x8000u;
 template <typename G>
+constexpr uint32_t row_shift() {
+    return 3u * G::kCells <= 255u ? 0u : 5u;
+}
+template <typename G>
 __host__ __device__ __forceinline__ uint32_t level_word(uint32_t partners, uint32_t i) {
+    static_assert(((3u * G::kCells) >> row_shift<G>()) <= 255u &&
+                      ((G::kCells >> row_shift<G>()) << row_shift<G>()) == G::kCells,
+                  "page rows must fit the level word");
     uint32_t n = 0, a0 = 0, a1 = 0, r0 = 0, r1 = 0;
 #pragma unroll
     for (uint32_t p = 0; p < 4; ++p) {
@@ -384,15 +398,15 @@ __host__ __device__ __forceinline__ uint32_t level_word(uint32_t partners, uint3
         if (o < i) {
             if (n == 0) {
                 a0 = o;
-                r0 = G::kCells * p;
+                r0 = (G::kCells * p) >> row_shift<G>();
             } else if (n == 1) {
                 a1 = o;
-                r1 = G::kCells * p;
+                r1 = (G::kCells * p) >> row_shift<G>();
             }
             ++n;
         }
     }
-    if (n == 0 || n > 2 || 3u * G::kCells > 255u) return kLevelGeneric;
+    if (n == 0 || n > 2) return kLevelGeneric;
     if (n == 1) {
         a1 = a0;
         r1 = r0;
@@ -451,9 +465,12 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
         const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[i];
         {
             const uint32_t a1 = lw & 0x7fu, a2 = (lw >> 8) & 0x7fu;
-            const uint32_t r1 = G::kPageOff + ((lw >> 16) & 0xffu), r2 = G::kPageOff + (lw >> 24);
+            const uint32_t r1 = G::kPageOff + (((lw >> 16) & 0xffu) << row_shift<G>()),
+                           r2 = G::kPageOff + ((lw >> 24) << row_shift<G>());
             uint32_t j1 = cyc_ld<G>(lds, cl, a1), j2 = cyc_ld<G>(lds, cl, a2);
-            Mask reach1 = static_cast<Mask>(1) << a1, reach2 = static_cast<Mask>(1) << a2;
+            Mask reach1 = 0, reach2 = 0;
+            or_bit(reach1, a1);
+            or_bit(reach2, a2);
             // A chain that closes walks its orbit again (back through the
             // anchor: the anchor's partner on this page is cell i, not yet
             // placed, so the walk re-enters the orbit), which only re-adds
@@ -470,7 +487,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             do {
                 // Two paired steps; each chain's two new cells join its
                 // reach in one three-input OR per dword.
-                Mask bits1[2], bits2[2];
+                [[maybe_unused]] Mask bits1[2], bits2[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const uint32_t h1 = hop<G>(r1, i, j1);
@@ -482,17 +499,24 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                     // bits (which costs a mask per step).
                     asm("" : "+v"(j1));
                     asm("" : "+v"(j2));
-                    bits1[u] = static_cast<Mask>(1) << h1;
-                    bits2[u] = static_cast<Mask>(1) << h2;
+                    if constexpr (sizeof(Mask) > 8) {
+                        or_bit(reach1, h1);
+                        or_bit(reach2, h2);
+                    } else {
+                        bits1[u] = static_cast<Mask>(1) << h1;
+                        bits2[u] = static_cast<Mask>(1) << h2;
+                    }
                     o1 = o1 && j1 != a1;
                     o2 = o2 && j2 != a2;
                 }
-                reach1 = or3(reach1, bits1[0], bits1[1]);
-                reach2 = or3(reach2, bits2[0], bits2[1]);
+                if constexpr (sizeof(Mask) == 8) {
+                    reach1 = or3(reach1, bits1[0], bits1[1]);
+                    reach2 = or3(reach2, bits2[0], bits2[1]);
+                }
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
             // cells below i reached by both walks
-            const Mask above = ~static_cast<Mask>(0) << i;
+            const Mask above = G::from(i);
             if (EVC) {
                 // Event counts as the one-page-at-a-time order: the second
                 // walk happens only if the first left good cells.
@@ -750,11 +774,15 @@ __device__ __forceinline__ bool stage_tables(const SymArgs &a, uint8_t *lds) {
     if constexpr (walk_mode<G>() == 2) {
         // The paired walk takes every level of the map from its level word:
         // a level that walks no page or more than two would send it round an
-        // orbit that never returns to its anchor.  (Levels 1..last of at most
-        // 64 cells, one per lane; wider tables have no paired level at all.)
+        // orbit that never returns to its anchor.  (Levels 1..last, one or
+        // two per lane.)
         const uint32_t c = tid & 63u;
-        const uint32_t lw = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff)[c];
-        bad = bad || __any(c >= 1u && c <= a.last && (lw & kLevelGeneric) != 0u);
+        const uint32_t *levels = reinterpret_cast<const uint32_t *>(lds + G::kLevelOff);
+        bool generic = c >= 1u && c <= a.last && (levels[c] & kLevelGeneric) != 0u;
+        if constexpr (G::kCells > 64) {
+            if (c + 64u < G::kCells) generic = generic || (c + 64u <= a.last && (levels[c + 64u] & kLevelGeneric) != 0u);
+        }
+        bad = bad || __any(generic);
     }
     if (bad && tid == 0) atomicOr(a.acc + ACC_ERROR, 4ull);
     return !bad;
@@ -1169,21 +1197,23 @@ hipError_t launch_features(const SymArgs &a, float *out, hipStream_t s) {
 }
 }  // namespace
 
-// The core that runs `core`'s request for the map in `a`: the u64 paired
-// walk (cores 0-2) needs every level 1..last to walk one or two pages, which
-// every spiral numbering gives; a map with any other level runs core 4 (one
-// page walk at a time, compact stack).  Core 3's two-lane walk and the 9x9
-// geometry already walk page by page.
-int sym_walk_core(int core, const SymArgs &a) {
+// The core that runs `core`'s request for the map in `a` (`cells` = n*n):
+// the paired walk (cores 0-2, and 9x9) needs every level 1..last to walk one
+// or two pages, which every spiral numbering gives; a map with any other
+// level runs one page walk at a time, core 4 up to 8x8 (compact stack) and
+// core 5 for 9x9.  Core 3's two-lane walk already walks page by page.
+int sym_walk_core(int core, const SymArgs &a, uint32_t cells) {
+    bool paired = true;
+    for (uint32_t i = 1; i <= a.last && i < 96u; ++i)
+        if ((level_word<G96>(a.edge_word[i], i) & kLevelGeneric) != 0u) paired = false;
+    if (cells > 64) return paired ? 2 : 5;
     if (core == 3 || core == 4) return core;
-    for (uint32_t i = 1; i <= a.last && i < 64u; ++i)
-        if ((level_word<G64>(a.edge_word[i], i) & kLevelGeneric) != 0u) return 4;
-    return core;
+    return paired ? core : 4;
 }
 
 hipError_t launch_sym_features(const SymArgs &a, float *out, hipStream_t s) {
     // Rows of 64 cells for every square up to 8x8.
-    return sym_walk_core(0, a) == 4 ? launch_features<G64Any>(a, out, s) : launch_features<G64>(a, out, s);
+    return sym_walk_core(0, a, 64) == 4 ? launch_features<G64Any>(a, out, s) : launch_features<G64>(a, out, s);
 }
 
 // Host-visible entry points (C++ linkage inside the library).
@@ -1197,8 +1227,8 @@ const void *pick(bool persistent, bool events) {
 }
 }  // namespace
 
-// core: 1 = wide u64 stack (up to 8x8), 3 = two lanes per subtree, 4 = one
-// page walk at a time (sym_walk_core), otherwise the compact 16-bit stack;
+// core: 1 = wide u64 stack (up to 8x8), 3 = two lanes per subtree, 4 / 5 =
+// one page walk at a time (sym_walk_core), otherwise the compact 16-bit stack;
 // cells = n*n picks the row size (49, 64 or, for 9x9, 81 entries).
 const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cells) {
     switch (geometry(core, cells)) {
@@ -1207,6 +1237,7 @@ const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cell
             return events ? reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, true>)
                           : reinterpret_cast<const void *>(&sym_persistent_kernel<G64Pair, false>);
         case Geo::k96: return pick<G96>(persistent, events);
+        case Geo::k96Any: return pick<G96Any>(persistent, events);
         case Geo::k64Wide: return pick<G64Wide>(persistent, events);
         case Geo::k64Any: return pick<G64Any>(persistent, events);
         case Geo::k49:
@@ -1263,6 +1294,7 @@ uint32_t sym_lds_bytes(int core, uint32_t levels, uint32_t block, uint32_t cells
     switch (geometry(core, cells)) {
         case Geo::k64Pair: return lds_of<G64Pair>(levels, block);
         case Geo::k96: return lds_of<G96>(levels, block);
+        case Geo::k96Any: return lds_of<G96Any>(levels, block);
         case Geo::k64Wide: return lds_of<G64Wide>(levels, block);
         case Geo::k64Any: return lds_of<G64Any>(levels, block);
         case Geo::k49: return lds_of<G49>(levels, block);
