@@ -29,7 +29,8 @@ const void *sym_kernel_ptr(int core, bool persistent, bool events, uint32_t cell
 hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t grid, uint32_t block,
                          uint32_t lds, bool events, hipStream_t);
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows);
-const void *lunnon_kernel_ptr(int core, bool persistent, bool events);
+const void *lunnon_kernel_ptr(int core, bool persistent, bool events, uint32_t dims);
+uint32_t lunnon_dims(const LunnonArgs &a);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
 int sym_walk_core(int core, const SymArgs &a, uint32_t cells);
@@ -701,7 +702,9 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         };
         const WorkItem *dev_items = static_cast<const WorkItem *>(f->items.ptr);
         if (ordered && (rc = reorder_items(f->host_items, opt->order, reordered, &dev_items)) != MF_OK) return rc;
-        rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events), lunnon_kernel_ptr(t.core, true, t.events), lds,
+        const uint32_t dims = lunnon_dims(a);
+        rc = run_solve(a, lunnon_kernel_ptr(t.core, false, t.events, dims), lunnon_kernel_ptr(t.core, true, t.events, dims),
+                       lds,
                        t, dev_items,
                        f->num, f->origins ? f->origins : f->num, per_count, per_nodes, per_map,
                        static_cast<uint32_t>(f->maps.size()), out, false,

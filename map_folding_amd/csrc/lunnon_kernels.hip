@@ -83,18 +83,22 @@ __device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m,
 // compiler track each lane's exit reason: 22 scalar and branch instructions
 // per step against 5 now (8 and 6 VALU), and the 6x6 depth-18 frontier ran in
 // 217 ms against 142 ms without it -- the walk was bound by scalar issue.
+// D > 0: every map of the launch has D dimensions (the loop over them is
+// unrolled; 0: each lane's map says).
+template <uint32_t D>
 __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
                                              uint32_t &steps, uint32_t &cands) {
     const uint32_t l = w.l;
     const uint64_t all = below_mask(l);
     uint64_t meet = all, reach_any = 0;
     bool constrained = false;
-    for (uint32_t i = 0; i < w.ndim; ++i) {
-        // Row l of dimension i in the triangular table: entries m <= l (the
-        // walk only looks up l itself and links of placed leaves, all < l).
-        const uint32_t row = w.tab + i * w.tri + ((l * (l + 1u)) >> 1);
+    // Row l of dimension i in the triangular table: entries m <= l (the walk
+    // only looks up l itself and links of placed leaves, all < l).
+    const uint32_t row0 = w.tab + ((l * (l + 1u)) >> 1);
+    auto dimension = [&](uint32_t i) {
+        const uint32_t row = row0 + i * w.tri;
         uint32_t m = lds[row + l];
-        if (m == l) continue;
+        if (m == l) return;
         constrained = true;
         uint64_t reach = 0;
         do {
@@ -104,6 +108,12 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         } while (m != l);
         meet &= reach;
         reach_any |= reach;
+    };
+    if constexpr (D > 0) {
+#pragma unroll
+        for (uint32_t i = 0; i < D; ++i) dimension(i);
+    } else {
+        for (uint32_t i = 0; i < w.ndim; ++i) dimension(i);
     }
     cands += constrained ? static_cast<uint32_t>(__popcll(reach_any)) : l;
     return constrained ? meet : all;
@@ -116,7 +126,7 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
 #endif
 constexpr uint32_t kLMaxPops = MF_LUNNON_MAX_POPS;
 
-template <typename Stack, bool EV>
+template <typename Stack, bool EV, uint32_t D>
 __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w) {
     constexpr uint32_t P = kLMaxPops;
     // The link entry and stack slot of leaf l - P: those of leaves l - P .. l
@@ -125,8 +135,12 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
     // the entries of leaf root - 1, inside the wave's rows, and rewrites its
     // link entry unchanged.)
     const uint32_t l0 = w.l;
-    const uint32_t rb = row_at<kLStride>(ln.cl - P * kLStride, l0);
-    const uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + l0 * Stack::kSlotBytes;
+    uint32_t rb = row_at<kLStride>(ln.cl - P * kLStride, l0);
+    uint32_t sb = (ln.s0 - P * Stack::kSlotBytes) + l0 * Stack::kSlotBytes;
+    // Opaque to the compiler, which would otherwise re-derive every pass's
+    // addresses from l0 instead of folding the offsets.
+    asm("" : "+v"(rb));
+    asm("" : "+v"(sb));
     // The child this step enters -- the node's first position, or after a
     // backtrack the next recorded sibling -- goes in by one shared block at
     // the end: its level's stack entry and its link.
@@ -135,7 +149,7 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
     uint32_t kid = 0;
     if (!w.popping) {
         uint32_t ns = 0, nc = 0;
-        uint64_t g = gap_mask(lds, ln, w, ns, nc);
+        uint64_t g = gap_mask<D>(lds, ln, w, ns, nc);
         if (w.floor == 0) {
             w.progressed = true;
             w.inodes += static_cast<uint32_t>(__popcll(g));
@@ -366,7 +380,7 @@ __device__ __forceinline__ void lspill_all(const LunnonArgs &a, uint8_t *lds, co
 
 }  // namespace
 
-template <typename Stack, bool EV>
+template <typename Stack, bool EV, uint32_t D>
 __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t span = tables_span(a.tables_bytes);
@@ -380,7 +394,7 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     w.popping = false;
     if (idx < a.num_items) lload_item<Stack>(lds, ln, a.items + idx, maps, a.row_entries, w);
     while (w.have) {
-        if (ldfs_step<Stack, EV>(lds, ln, w)) {
+        if (ldfs_step<Stack, EV, D>(lds, ln, w)) {
             lretire(a, w, tot);
             w.have = false;
         } else {
@@ -390,7 +404,7 @@ __global__ __launch_bounds__(256) void lunnon_static_kernel(LunnonArgs a) {
     lflush(a, threadIdx.x & 63u, tot);
 }
 
-template <typename Stack, bool EV>
+template <typename Stack, bool EV, uint32_t D>
 __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     // Graph rounds read their item count from the previous round's spill
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
         if (!__any(w.have)) break;
 
         for (uint32_t s = 0; s < a.budget && w.have; ++s) {
-            if (ldfs_step<Stack, EV>(lds, ln, w)) {
+            if (ldfs_step<Stack, EV, D>(lds, ln, w)) {
                 lretire(a, w, tot);
                 w.have = false;
             }
@@ -453,7 +467,7 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
                 // Progress guarantee: every lane completes one counted gap
                 // computation of its item before its remaining work is split.
                 while (__any(w.have && !w.progressed)) {
-                    if (w.have && !w.progressed && ldfs_step<Stack, EV>(lds, ln, w)) {
+                    if (w.have && !w.progressed && ldfs_step<Stack, EV, D>(lds, ln, w)) {
                         lretire(a, w, tot);
                         w.have = false;
                     }
@@ -467,12 +481,12 @@ __global__ __launch_bounds__(256) void lunnon_persistent_kernel(LunnonArgs a) {
 }
 
 namespace {
-template <typename Stack>
+template <typename Stack, uint32_t D>
 const void *lpick(bool persistent, bool events) {
-    if (persistent) return events ? reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, true>)
-                                  : reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, false>);
-    return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, true>)
-                  : reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, false>);
+    if (persistent) return events ? reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, true, D>)
+                                  : reinterpret_cast<const void *>(&lunnon_persistent_kernel<Stack, false, D>);
+    return events ? reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, true, D>)
+                  : reinterpret_cast<const void *>(&lunnon_static_kernel<Stack, false, D>);
 }
 }  // namespace
 
@@ -481,15 +495,26 @@ uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint
     return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (((rows * kLStride + 15u) & ~15u) + levels * slot);
 }
 
-// core: 1 = wide u64 stack, otherwise the compact 16-bit stack.
-const void *lunnon_kernel_ptr(int core, bool persistent, bool events) {
-    return core == 1 ? lpick<WideStack>(persistent, events) : lpick<CompactStack>(persistent, events);
+// 2 when every map of the launch is two-dimensional (every rectangle),
+// which selects the instance with the dimension loop unrolled; else 0.
+uint32_t lunnon_dims(const LunnonArgs &a) {
+    for (uint32_t k = 0; k < a.num_maps; ++k)
+        if (a.maps[k].ndim != 2) return 0;
+    return 2;
+}
+
+// core: 1 = wide u64 stack, otherwise the compact 16-bit stack; dims as
+// lunnon_dims gives it.
+const void *lunnon_kernel_ptr(int core, bool persistent, bool events, uint32_t dims) {
+    if (core == 1) return lpick<WideStack, 0>(persistent, events);
+    return dims == 2 ? lpick<CompactStack, 2>(persistent, events) : lpick<CompactStack, 0>(persistent, events);
 }
 
 hipError_t launch_lunnon(const LunnonArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,
                          uint32_t lds_bytes, bool events, hipStream_t s) {
     void *args[] = {const_cast<LunnonArgs *>(&a)};
-    return hipLaunchKernel(lunnon_kernel_ptr(core, persistent, events), dim3(grid), dim3(block), args, lds_bytes, s);
+    return hipLaunchKernel(lunnon_kernel_ptr(core, persistent, events, lunnon_dims(a)), dim3(grid), dim3(block), args,
+                           lds_bytes, s);
 }
 
 }  // namespace mf
