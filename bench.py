@@ -383,8 +383,8 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
             t1 = time.time()
             cfg = autotune.tune_resplit(front, budgets=(32, 64), tail_mult=(1,), waves=(16, 32), log=log)
             batched = mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["tail"],
-                                      waves_per_cu=cfg["waves_per_cu"])
-            tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "kernel_ms")}
+                                      waves_per_cu=cfg["waves_per_cu"], core=cfg["core"])
+            tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "core", "kernel_ms")}
             tuned.update({"trials": len(cfg["trials"]), "tune_s": time.time() - t1})
 
         def solve(opt):
@@ -402,9 +402,11 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
             cfg = autotune.tune_lunnon_batch(shapes, targets=(256, 1024, 4096, 16384), budgets=(32, 64, 128),
                                              waves=(16, 32), log=log, extra_depths=[depths])
             depths = cfg["depths"]
-            batched = mf.SolveOptions(count_events=False, budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"])
+            batched = mf.SolveOptions(count_events=False, budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"],
+                                      core=cfg["core"])
             tuned = {"depths": depths, "budget": cfg["budget"], "waves_per_cu": cfg["waves_per_cu"],
-                     "kernel_ms": cfg["kernel_ms"], "trials": len(cfg["trials"]), "tune_s": time.time() - t1}
+                     "core": cfg["core"], "kernel_ms": cfg["kernel_ms"], "trials": len(cfg["trials"]),
+                     "tune_s": time.time() - t1}
         batch = mf.LunnonBatchFrontier(shapes, depths)
         label = f"64 rectangles (seed {args.seed}), per-map frontier depths {min(depths)}..{max(depths)}"
         items = len(shapes)

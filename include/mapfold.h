@@ -102,11 +102,13 @@ typedef struct {
     int32_t tail;             /* re-split: iterations a lane may still run once the input is
                                  drained before it splits its remaining work */
     int32_t count_events;     /* 1: count walks/steps too (nodes/leaves always counted) */
-    int32_t core;             /* sym DFS stack: 0 = default (compact), 1 = one u64 mask per
-                                 level, 2 = compact 16-bit entries with recompute, 3 = compact
-                                 with two lanes per subtree (a lane pair shares one row and
+    int32_t core;             /* DFS stack and walk: 0 = default (compact), 1 = one u64 mask
+                                 per level, 2 = compact 16-bit entries with recompute, 3 = compact
+                                 with two lanes per subtree (sym: a lane pair shares one row and
                                  stack and splits each node's page walks; re-split schedule,
-                                 up to 8x8) */
+                                 up to 8x8), 4 = compact with unpaired walks (sym: one page walk
+                                 at a time; Lunnon: one dimension at a time, where the default
+                                 walks the two dimensions of two-dimensional maps in one loop) */
     int32_t graph;            /* re-split: 0 = default (on), 1 = on, 2 = off.  On: after the
                                  first round, rounds run as a captured HIP graph of 8 rounds
                                  (device-side item counts), replayed until a round spills

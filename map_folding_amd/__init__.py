@@ -51,8 +51,9 @@ class SolveOptions:
     budget: int = 0
     tail: int = 0
     count_events: bool = True
-    core: int = 0                     # sym stack: 0 default, 1 wide u64 masks, 2 compact 16-bit,
-                                      # 3 compact with two lanes per subtree (re-split, up to 8x8)
+    core: int = 0                     # stack and walk: 0 default, 1 wide u64 masks, 2 compact 16-bit,
+                                      # 3 compact with two lanes per subtree (sym, re-split, up to 8x8),
+                                      # 4 compact with unpaired walks (one page / one dimension at a time)
     queue_capacity: int = 0
     order: np.ndarray | None = None   # permutation of item indices (learned order)
     graph: int = 0                    # re-split rounds as a HIP graph: 0 default (on), 1 on, 2 off

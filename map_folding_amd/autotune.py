@@ -54,19 +54,21 @@ def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 25
                       repeats=2, log=None, extra_depths=()):
     """Pick per-map frontier depths (by a target states-per-map, or one of the
     explicit per-map depth lists in `extra_depths`), then the re-split budget
-    and resident waves, for one batched launch of `shapes`.
+    and resident waves, then whether the two dimensions of the (rectangular)
+    maps walk in one loop (the default) or one after the other (core 4), for
+    one batched launch of `shapes`.
 
-    Returns {"depths", "budget", "waves_per_cu", "kernel_ms", "trials"}."""
+    Returns {"depths", "budget", "waves_per_cu", "core", "kernel_ms", "trials"}."""
     say = log or (lambda m: None)
     trials = []
     ref_totals = None
 
-    def solve_with(depths, budget, wv):
+    def solve_with(depths, budget, wv, core=0):
         nonlocal ref_totals
         t0 = time.time()
         b = LunnonBatchFrontier(shapes, depths)
         gen_s = time.time() - t0
-        opt = SolveOptions(count_events=False, budget=budget, waves_per_cu=wv)
+        opt = SolveOptions(count_events=False, budget=budget, waves_per_cu=wv, core=core)
 
         def once():
             totals, r = b.solve_maps(opt)
@@ -77,9 +79,10 @@ def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 25
             ref_totals = totals
         elif totals != ref_totals:
             raise RuntimeError("a tuning trial changed a map total")
-        trials.append({"depths": list(depths), "budget": budget, "waves_per_cu": wv, "kernel_ms": ms,
+        trials.append({"depths": list(depths), "budget": budget, "waves_per_cu": wv, "core": core, "kernel_ms": ms,
                        "gen_s": gen_s})
-        say(f"  depths {min(depths)}..{max(depths)} budget {budget} waves {wv}: {ms:.1f} ms (gen {gen_s:.2f}s)")
+        say(f"  depths {min(depths)}..{max(depths)} budget {budget} waves {wv} core {core}: {ms:.1f} ms "
+            f"(gen {gen_s:.2f}s)")
         return ms
 
     best = None
@@ -99,15 +102,21 @@ def tune_lunnon_batch(shapes, targets=(64, 256, 1024, 4096), budgets=(16, 64, 25
             ms = solve_with(depths, bu, wv)
             if ms < best_cfg[0]:
                 best_cfg = (ms, bu, wv)
-    return {"depths": depths, "budget": best_cfg[1], "waves_per_cu": best_cfg[2], "kernel_ms": best_cfg[0],
-            "trials": trials}
+    core = 0
+    ms = solve_with(depths, best_cfg[1], best_cfg[2], core=4)
+    if ms < best_cfg[0]:
+        best_cfg, core = (ms, best_cfg[1], best_cfg[2]), 4
+    return {"depths": depths, "budget": best_cfg[1], "waves_per_cu": best_cfg[2], "core": core,
+            "kernel_ms": best_cfg[0], "trials": trials}
 
 
 def tune_resplit(frontier, budgets=(32, 64, 128), tail_mult=(1, 4), waves=(20, 32), repeats=1, log=None):
     """Budget, tail (as a multiple of the budget) and resident waves of the
-    persistent re-split schedule on a resident frontier.
+    persistent re-split schedule on a resident frontier, then paired walks
+    (the default) against unpaired ones (core 4: one page or one dimension at
+    a time).
 
-    Returns {"budget", "tail", "waves_per_cu", "kernel_ms", "trials"}."""
+    Returns {"budget", "tail", "waves_per_cu", "core", "kernel_ms", "trials"}."""
     say = log or (lambda m: None)
     trials = []
     ref = None
@@ -128,7 +137,21 @@ def tune_resplit(frontier, budgets=(32, 64, 128), tail_mult=(1, 4), waves=(20, 3
                 trials.append({"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "kernel_ms": ms})
                 say(f"  budget {bu} tail {bu * tm} waves {wv}: {ms:.1f} ms")
                 if best is None or ms < best["kernel_ms"]:
-                    best = {"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "kernel_ms": ms}
+                    best = {"budget": bu, "tail": bu * tm, "waves_per_cu": wv, "core": 0, "kernel_ms": ms}
+    opt = SolveOptions(count_events=False, budget=best["budget"], tail=best["tail"], waves_per_cu=best["waves_per_cu"],
+                       core=4)
+
+    def unpaired():
+        r, _, _ = frontier.solve(opt)
+        return r.kernel_ms, r.count
+    ms, count = _best(unpaired, repeats)
+    if count != ref:
+        raise RuntimeError("a tuning trial changed the count")
+    trials.append({"budget": best["budget"], "tail": best["tail"], "waves_per_cu": best["waves_per_cu"], "core": 4,
+                   "kernel_ms": ms})
+    say(f"  unpaired walks (core 4): {ms:.1f} ms")
+    if ms < best["kernel_ms"]:
+        best.update(core=4, kernel_ms=ms)
     best["trials"] = trials
     return best
 

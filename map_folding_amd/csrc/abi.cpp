@@ -31,6 +31,7 @@ hipError_t launch_lunnon(const LunnonArgs &, int core, bool persistent, uint32_t
 uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint32_t block, uint32_t rows);
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events, uint32_t dims);
 uint32_t lunnon_dims(const LunnonArgs &a);
+bool lunnon_paired(int core, const LunnonArgs &a);
 hipError_t launch_round_reset(uint32_t *claim, uint32_t *out_count, hipStream_t s);
 const void *sym_deepen_ptr(int core, bool events, uint32_t cells);
 int sym_walk_core(int core, const SymArgs &a, uint32_t cells);
@@ -130,7 +131,7 @@ constexpr int kDefaultCore = 2;   // compact 16-bit stack
 Tuned tune(const mf_solve_options *opt) {
     Tuned t;
     t.schedule = opt ? opt->schedule : MF_SCHED_RESPLIT;
-    t.core = (opt && opt->core >= 1 && opt->core <= 3) ? opt->core : kDefaultCore;
+    t.core = (opt && opt->core >= 1 && opt->core <= 4) ? opt->core : kDefaultCore;
     t.block = (opt && opt->block > 0) ? static_cast<uint32_t>(opt->block) : 256u;
     t.block = std::max(64u, std::min(256u, (t.block / 64u) * 64u));
     t.waves_per_cu = (opt && opt->waves_per_cu > 0) ? static_cast<uint32_t>(opt->waves_per_cu) : 32u;
@@ -444,7 +445,9 @@ int lunnon_args(const mf_frontier &f, LunnonArgs &a, std::vector<uint8_t> &blob,
     a.num_maps = static_cast<uint32_t>(f.maps.size());
     a.tables_bytes = static_cast<uint32_t>(blob.size());
     // Leaves 0..n-1 of the largest map: leaf n is never linked (inserting it
-    // counts completions), so a row needs n entries.
+    // counts completions), so a row needs n entries (n + 1 when the launch
+    // walks both dimensions in one loop, which points a leaf's own entry at
+    // itself while it computes the leaf's gaps: solve_frontier).
     a.row_entries = 0;
     for (const LunnonProblem &pr : f.maps) a.row_entries = std::max<uint32_t>(a.row_entries, pr.n);
     a.levels = lv;
@@ -601,7 +604,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         sym_args(*f, a, &levels);
         // Numberings whose levels do not all walk one or two pages run one
         // page walk at a time.
-        const int core = sym_walk_core(c96 ? 2 : t.core, a, static_cast<uint32_t>(f->sym.n2));
+        const int core = sym_walk_core(c96 && t.core != 4 ? 2 : t.core, a, static_cast<uint32_t>(f->sym.n2));
         const uint32_t cells = static_cast<uint32_t>(f->sym.n2);
         const uint32_t lds = sym_lds_bytes(core, levels, t.block, cells);
         auto launch = [&](const SymArgs &args, bool persistent, uint32_t grid) {
@@ -692,6 +695,7 @@ int solve_frontier(mf_frontier *f, const mf_solve_options *opt, uint64_t *per_co
         HIP_TRY(hipMemcpyAsync(g.tables.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream));
         a.tables = static_cast<const uint8_t *>(g.tables.ptr);
         a.per_map = nullptr;
+        if (lunnon_paired(t.core, a)) ++a.row_entries;   // at most 64: paired launches have n <= 63
         const uint32_t lds = lunnon_lds_bytes(t.core, a.tables_bytes, levels, t.block, a.row_entries);
         if (per_map) {
             if ((rc = g.per_map.ensure(f->maps.size() * 8)) != MF_OK) return rc;

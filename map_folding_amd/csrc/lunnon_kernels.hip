@@ -83,13 +83,52 @@ __device__ __forceinline__ void unlink(uint8_t *lds, const Lane &ln, uint32_t m,
 // compiler track each lane's exit reason: 22 scalar and branch instructions
 // per step against 5 now (8 and 6 VALU), and the 6x6 depth-18 frontier ran in
 // 217 ms against 142 ms without it -- the walk was bound by scalar issue.
-// D > 0: every map of the launch has D dimensions (the loop over them is
-// unrolled; 0: each lane's map says).
+// How a gap computation walks the dimensions: kDimsAny, each lane's map
+// gives them, walked one after the other; kDims2, every map of the launch is
+// two-dimensional (lunnon_dims), the two walked one after the other
+// (unrolled); kDims2Paired, the same maps with at most 63 leaves, both
+// dimensions walked in one loop.
+enum : uint32_t { kDimsAny = 0, kDims2 = 1, kDims2Paired = 2 };
 template <uint32_t D>
-__device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
+__device__ __forceinline__ uint64_t gap_mask(uint8_t *__restrict__ lds, const Lane &ln, LWork &w,
                                              uint32_t &steps, uint32_t &cands) {
     const uint32_t l = w.l;
     const uint64_t all = below_mask(l);
+    if constexpr (D == kDims2Paired) {
+        const uint32_t r0 = w.tab + ((l * (l + 1u)) >> 1), r1 = r0 + w.tri;
+        uint32_t m0 = lds[r0 + l], m1 = lds[r1 + l];
+        const bool c0 = m0 != l, c1 = m1 != l;
+        // Leaf l is not linked yet, so its own link entry (row entry l <= 63)
+        // is free: while its gaps are computed it points at l.  A walk back
+        // at l then steps to d[i][l][l], its first leaf, and walks its orbit
+        // again, which adds nothing to its reach but bit l (outside `all`);
+        // an unconstrained dimension stays at l.  So a closed walk needs no
+        // mask, and the two walks' LDS loads are in flight together.
+        row_st<kLStride>(lds, row_at<kLStride>(ln.cl, l), l);
+        uint64_t reach0 = 0, reach1 = 0;
+        bool o0 = c0, o1 = c1;
+        // Two steps per trip: a closed walk's extra step changes nothing,
+        // so the exit is tested every second step.
+        do {
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                reach0 |= 1ull << m0;
+                reach1 |= 1ull << m1;
+                m0 = lds[r0 + row_ld<kLStride>(lds, row_at<kLStride>(ln.cl, m0))];
+                m1 = lds[r1 + row_ld<kLStride>(lds, row_at<kLStride>(ln.cl, m1))];
+                o0 = o0 && m0 != l;
+                o1 = o1 && m1 != l;
+            }
+        } while (o0 || o1);
+        // A walk visits each leaf of its orbit once: its step count is the
+        // orbit's size (leaves below l).
+        reach0 = c0 ? reach0 & all : 0;
+        reach1 = c1 ? reach1 & all : 0;
+        steps += mask_popcount(reach0) + mask_popcount(reach1);
+        const bool constrained = c0 || c1;
+        cands += constrained ? mask_popcount(reach0 | reach1) : l;
+        return (c0 ? reach0 : all) & (c1 ? reach1 : all);
+    }
     uint64_t meet = all, reach_any = 0;
     bool constrained = false;
     // Row l of dimension i in the triangular table: entries m <= l (the walk
@@ -109,9 +148,9 @@ __device__ __forceinline__ uint64_t gap_mask(const uint8_t *__restrict__ lds, co
         meet &= reach;
         reach_any |= reach;
     };
-    if constexpr (D > 0) {
-#pragma unroll
-        for (uint32_t i = 0; i < D; ++i) dimension(i);
+    if constexpr (D == kDims2) {
+        dimension(0);
+        dimension(1);
     } else {
         for (uint32_t i = 0; i < w.ndim; ++i) dimension(i);
     }
@@ -495,19 +534,29 @@ uint32_t lunnon_lds_bytes(int core, uint32_t tables_bytes, uint32_t levels, uint
     return ((tables_bytes + 15u) & ~15u) + kMapBytes + (block / 64u) * (((rows * kLStride + 15u) & ~15u) + levels * slot);
 }
 
-// 2 when every map of the launch is two-dimensional (every rectangle),
-// which selects the instance with the dimension loop unrolled; else 0.
+// 2 when every map of the launch is two-dimensional (every rectangle) with
+// at most 63 leaves, 1 when they are two-dimensional with more, else 0.
 uint32_t lunnon_dims(const LunnonArgs &a) {
-    for (uint32_t k = 0; k < a.num_maps; ++k)
+    uint32_t d = 2;
+    for (uint32_t k = 0; k < a.num_maps; ++k) {
         if (a.maps[k].ndim != 2) return 0;
-    return 2;
+        if (a.maps[k].n > 63) d = 1;
+    }
+    return d;
 }
 
-// core: 1 = wide u64 stack, otherwise the compact 16-bit stack; dims as
-// lunnon_dims gives it.
+// Whether `core` runs the launch with both dimensions walked in one loop
+// (its rows then need entry n too: lunnon_args).
+bool lunnon_paired(int core, const LunnonArgs &a) { return core != 1 && core != 4 && lunnon_dims(a) == 2; }
+
+// core: 1 = wide u64 stack; 4 = compact stack, dimensions walked one after
+// the other; otherwise compact, two-dimensional launches walking both
+// dimensions in one loop.  dims as lunnon_dims gives it.
 const void *lunnon_kernel_ptr(int core, bool persistent, bool events, uint32_t dims) {
-    if (core == 1) return lpick<WideStack, 0>(persistent, events);
-    return dims == 2 ? lpick<CompactStack, 2>(persistent, events) : lpick<CompactStack, 0>(persistent, events);
+    if (core == 1) return lpick<WideStack, kDimsAny>(persistent, events);
+    if (dims == 0) return lpick<CompactStack, kDimsAny>(persistent, events);
+    if (dims == 1 || core == 4) return lpick<CompactStack, kDims2>(persistent, events);
+    return lpick<CompactStack, kDims2Paired>(persistent, events);
 }
 
 hipError_t launch_lunnon(const LunnonArgs &a, int core, bool persistent, uint32_t grid, uint32_t block,

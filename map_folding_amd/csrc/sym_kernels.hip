@@ -1200,14 +1200,15 @@ hipError_t launch_features(const SymArgs &a, float *out, hipStream_t s) {
 // The core that runs `core`'s request for the map in `a` (`cells` = n*n):
 // the paired walk (cores 0-2, and 9x9) needs every level 1..last to walk one
 // or two pages, which every spiral numbering gives; a map with any other
-// level runs one page walk at a time, core 4 up to 8x8 (compact stack) and
-// core 5 for 9x9.  Core 3's two-lane walk already walks page by page.
+// level, or a request for core 4, runs one page walk at a time: core 4 up to
+// 8x8 (compact stack), core 5 for 9x9.  Core 3's two-lane walk already walks
+// page by page.
 int sym_walk_core(int core, const SymArgs &a, uint32_t cells) {
-    bool paired = true;
+    bool paired = core != 4;
     for (uint32_t i = 1; i <= a.last && i < 96u; ++i)
         if ((level_word<G96>(a.edge_word[i], i) & kLevelGeneric) != 0u) paired = false;
     if (cells > 64) return paired ? 2 : 5;
-    if (core == 3 || core == 4) return core;
+    if (core == 3) return core;
     return paired ? core : 4;
 }
 

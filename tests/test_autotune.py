@@ -27,9 +27,12 @@ def test_tune_lunnon_batch_keeps_totals(gpu, golden):
     assert [t["depths"] for t in cfg["trials"][:3]][2] == extra[0]
     with pytest.raises(ValueError):
         autotune.tune_lunnon_batch(shapes, targets=(), budgets=(16,), waves=(32,), repeats=1, extra_depths=[[6]])
+    # The last trial walks the dimensions one after the other (core 4).
+    assert cfg["trials"][-1]["core"] == 4 and cfg["core"] in (0, 4)
     b = gpu.LunnonBatchFrontier(shapes, cfg["depths"])
-    totals, _ = b.solve_maps(gpu.SolveOptions(budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"]))
-    assert totals == [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
+    for core in (0, 4):
+        totals, _ = b.solve_maps(gpu.SolveOptions(budget=cfg["budget"], waves_per_cu=cfg["waves_per_cu"], core=core))
+        assert totals == [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
     b.close()
 
 
@@ -40,8 +43,10 @@ def test_tune_resplit_keeps_count(gpu, golden):
     f = gpu.SymFrontier(7, 24, cyc, mul)
     ref, _, _ = f.solve()
     cfg = autotune.tune_resplit(f, budgets=(16, 64), tail_mult=(1, 4), waves=(20, 32))
-    assert len(cfg["trials"]) == 8 and cfg["kernel_ms"] == min(t["kernel_ms"] for t in cfg["trials"])
-    r, _, _ = f.solve(gpu.SolveOptions(budget=cfg["budget"], tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"]))
+    assert len(cfg["trials"]) == 9 and cfg["kernel_ms"] == min(t["kernel_ms"] for t in cfg["trials"])
+    assert cfg["trials"][-1]["core"] == 4
+    r, _, _ = f.solve(gpu.SolveOptions(budget=cfg["budget"], tail=cfg["tail"], waves_per_cu=cfg["waves_per_cu"],
+                                       core=cfg["core"]))
     assert r.count == ref.count
     f.close()
 
@@ -82,3 +87,19 @@ def test_tune_sym_stages_and_lanes():
     f3 = _FakeSymFrontier(ms)
     assert autotune.tune_sym(f3, level=36, deepens=(46,), lanes=(1,))["lanes"] == 1
     assert all(o.core == 0 for o in f3.seen)
+
+
+def test_tune_resplit_unpaired_stage():
+    """After the budget / tail / waves sweep, one trial runs unpaired walks
+    (core 4) at the chosen schedule and is kept only when it is faster."""
+    from map_folding_amd import autotune
+
+    def ms(opt):
+        return 50.0 + (3.0 if opt.budget == 16 else 0.0) + (4.0 if opt.core == 4 else 0.0)
+    f = _FakeSymFrontier(ms)
+    cfg = autotune.tune_resplit(f, budgets=(16, 64), tail_mult=(1,), waves=(32,))
+    assert (cfg["budget"], cfg["core"], cfg["kernel_ms"]) == (64, 0, 50.0)
+    assert f.seen[-1].core == 4 and f.seen[-1].budget == 64
+    f2 = _FakeSymFrontier(lambda o: ms(o) - (8.0 if o.core == 4 else 0.0))
+    cfg2 = autotune.tune_resplit(f2, budgets=(16, 64), tail_mult=(1,), waves=(32,))
+    assert (cfg2["core"], cfg2["kernel_ms"]) == (4, 46.0)

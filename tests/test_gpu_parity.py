@@ -20,7 +20,7 @@ SCHEDULES = {
     "static": dict(schedule=0),
     "resplit": dict(schedule=1),
     "resplit_tight": dict(schedule=1, budget=8, tail=8),
-    # the wide u64-mask sym stack (default is the compact one; Lunnon ignores `core`)
+    # the wide u64-mask stack (default is the compact one)
     "static_wide": dict(schedule=0, core=1),
     "resplit_wide": dict(schedule=1, core=1),
     "resplit_wide_tight": dict(schedule=1, core=1, budget=4, tail=4),
@@ -31,8 +31,13 @@ SCHEDULES = {
     # two lanes per subtree (sym, up to 8x8; re-split only)
     "resplit_pair": dict(schedule=1, core=3),
     "resplit_pair_tight": dict(schedule=1, core=3, budget=4, tail=4),
+    # unpaired walks (sym: one page at a time; Lunnon: the two dimensions of a
+    # rectangle one after the other instead of in one loop)
+    "static_unpaired": dict(schedule=0, core=4),
+    "resplit_unpaired": dict(schedule=1, core=4),
 }
-LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight", "resplit_tight_nograph"]
+LUNNON_SCHEDULES = ["static", "resplit", "resplit_tight", "resplit_tight_nograph", "static_unpaired",
+                    "resplit_unpaired", "resplit_wide"]
 
 
 def _opts(mf, name, **kw):
@@ -121,7 +126,7 @@ def test_lunnon_batch_bench_draw_full_size(gpu, golden):
     depths = [bench.rect_depth(p, q) for p, q in shapes]
     _say(f"lunnon batch: the 64-rectangle draw of seed 0, depths {min(depths)}..{max(depths)}")
     front = gpu.LunnonBatchFrontier(shapes, depths)
-    for sched in ("resplit", "resplit_nograph"):
+    for sched in ("resplit", "resplit_nograph", "resplit_unpaired"):
         totals, res = front.solve_maps(_opts(gpu, sched, count_events=False))
         for (p, q), t in zip(shapes, totals):
             assert t == golden["rectangles"][f"{p}x{q}"], (p, q, sched)
@@ -142,7 +147,7 @@ def _chunk(golden, n, e, start, mx):
 
 @pytest.mark.parametrize("sched", ["resplit", "resplit_tight", "resplit_tiny", "resplit_wide",
                                    "resplit_wide_tight", "resplit_nograph", "resplit_tight_nograph",
-                                   "resplit_pair", "resplit_pair_tight"])
+                                   "resplit_pair", "resplit_pair_tight", "resplit_unpaired"])
 def test_sym_8x8_first_300_events(gpu, golden, sched):
     _say(f"sym 8x8 d36 [0,300) schedule={sched}")
     g = _chunk(golden, 8, 36, 0, 300)
