@@ -377,7 +377,7 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
         want = golden["square_totals"][str(side)]
         label = f"{side}x{side} Lunnon DFS, partition depth {depth} ({states.shape[0]} states)"
         items = states.shape[0]
-        batched = mf.SolveOptions(count_events=False)
+        batched = mf.SolveOptions(count_events=False, budget=args.budget, tail=args.budget, waves_per_cu=args.waves)
         if not args.no_tune:
             from map_folding_amd import autotune
             t1 = time.time()
@@ -395,7 +395,7 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
         depths = [rect_depth(p, q) for p, q in shapes]
         want_maps = [golden["rectangles"][f"{p}x{q}"] for p, q in shapes]
         want = sum(want_maps)
-        batched = mf.SolveOptions(count_events=False)
+        batched = mf.SolveOptions(count_events=False, budget=args.budget, waves_per_cu=args.waves)
         if not args.no_tune:
             from map_folding_amd import autotune
             t1 = time.time()
@@ -539,7 +539,9 @@ def main():
     ap.add_argument("--theta", type=float, default=None,
                     help="sym with --no-tune: learned threshold (log2 predicted nodes); unset = no learned targets")
     ap.add_argument("--cap", type=int, default=4, help="sym with --no-tune: most levels the model adds")
-    ap.add_argument("--budget", type=int, default=64, help="sym with --no-tune: re-split budget")
+    ap.add_argument("--budget", type=int, default=64, help="with --no-tune: re-split budget")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="Lunnon workloads with --no-tune: resident waves per CU (0: the library default)")
     ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
                     help="sym with --no-tune: lanes per subtree (2: a lane pair splits each node's page walks)")
     ap.add_argument("--no-compare", action="store_true",
