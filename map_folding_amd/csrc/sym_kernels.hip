@@ -615,7 +615,9 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
     // does the item's root.  Each pass is one region with selects inside: a
     // lane at its root rewrites its own cycle entry unchanged instead of
     // branching.
-    bool fin = false, got = false;
+    // Only `pop` and VGPR values cross the passes: whether the pops stopped
+    // at a level with more children (ae != 0: such an entry is nonzero) and
+    // whether they reached the root are read off afterwards.
     typename Stack::Entry ae = 0;
 #pragma unroll
     for (uint32_t k = 0; k < P; ++k) {
@@ -628,16 +630,17 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
             cyc_st<G>(lds, ln.cl, root ? w.i - 1 : Stack::child(e), c);   // unplace (at the root: unchanged)
             const bool more = !root && Stack::any_more(e);
             ae = more ? e : ae;
-            got = more;
             w.i = root ? w.i : w.i - 1;
-            fin = root;
             pop = !(root || more);
         }
     }
+    // A lane that popped and stopped without a level with more children
+    // passed its root: the item is finished.
+    const bool fin = !enter && !pop && ae == 0;
     // Stopped at a level with more children: its recorded next sibling, or a
     // recompute of the node above the current child.
-    const bool nx = got && Stack::has_next(ae);
-    if (got && !nx) w.floor = Stack::child(ae) + 1;
+    const bool nx = Stack::has_next(ae);
+    if (Stack::recompute(ae)) w.floor = Stack::child(ae) + 1;
     if (nx) {
         ent = Stack::advance(ae);
         kid = Stack::child(ent);
