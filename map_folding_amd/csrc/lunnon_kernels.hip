@@ -219,7 +219,7 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
     // below its position.  A level with more positions (recorded or not)
     // stops it, and so does the item's root.  Each pass is one region with
     // selects inside.
-    bool fin = false, got = false;
+    // Only `pop` and VGPR values cross the passes (as in the symmetry DFS).
     typename Stack::Entry ae = 0;
 #pragma unroll
     for (uint32_t k = 0; k < P; ++k) {
@@ -233,16 +233,16 @@ __device__ __forceinline__ bool ldfs_step(uint8_t *lds, const Lane &ln, LWork &w
             row_st<kLStride>(lds, root ? at : row_at<kLStride>(ln.cl, Stack::child(e)), c);
             const bool more = !root && Stack::any_more(e);
             ae = more ? e : ae;
-            got = more;
             w.l = root ? w.l : w.l - 1;
-            fin = root;
             pop = !(root || more);
         }
     }
+    // Popped and stopped without a level with more positions: past the root.
+    const bool fin = !enter && !pop && ae == 0;
     // Stopped at a level with more positions: its recorded next sibling, or
     // a recompute of the leaf's gaps above the current position.
-    const bool nx = got && Stack::has_next(ae);
-    if (got && !nx) w.floor = Stack::child(ae) + 1;
+    const bool nx = Stack::has_next(ae);
+    if (Stack::recompute(ae)) w.floor = Stack::child(ae) + 1;
     if (nx) {
         ent = Stack::advance(ae);
         kid = Stack::child(ent);
