@@ -116,10 +116,16 @@ __device__ __forceinline__ void row_store_n(const uint8_t *lds, uint32_t lb, uin
 
 // Lowest set bit of a u64 or u128 mask (nonzero).
 __device__ __forceinline__ uint32_t mask_ctz(uint64_t v) { return static_cast<uint32_t>(__builtin_ctzll(v)); }
+// The same for any mask: an empty one gives the top bit's index (for values
+// computed by lanes that do not use them).
+__device__ __forceinline__ uint32_t mask_ctz_any(uint64_t v) { return mask_ctz(v | (1ull << 63)); }
 __device__ __forceinline__ uint32_t mask_ctz(unsigned __int128 v) {
     const uint64_t lo = static_cast<uint64_t>(v);
     return lo ? static_cast<uint32_t>(__builtin_ctzll(lo))
               : 64u + static_cast<uint32_t>(__builtin_ctzll(static_cast<uint64_t>(v >> 64)));
+}
+__device__ __forceinline__ uint32_t mask_ctz_any(unsigned __int128 v) {
+    return mask_ctz(v | (static_cast<unsigned __int128>(1) << 127));
 }
 __device__ __forceinline__ uint32_t mask_popcount(uint64_t v) { return static_cast<uint32_t>(__popcll(v)); }
 __device__ __forceinline__ uint32_t mask_popcount(unsigned __int128 v) {
@@ -225,11 +231,13 @@ struct CompactStackT {
         const uint32_t j2 = mask_ctz(g & (g - 1));
         return j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);
     }
-    // The same with the lowest child j (ctz of g) known too.
+    // The same with the lowest child j (ctz of g) known too, without
+    // branches (a wave runs both sides anyway): with one child the second
+    // ctz reads an empty mask and is not used.
     static __device__ __forceinline__ uint32_t encode_at(uint32_t j, Mask g, uint32_t count) {
-        if (count == 1) return j;
-        const uint32_t j2 = mask_ctz(g & (g - 1));
-        return j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);
+        const uint32_t j2 = mask_ctz_any(g & (g - 1)) & kField;
+        const uint32_t two = j | (j2 << B) | kNext | (count > 2 ? kMore : 0u);
+        return count == 1 ? j : two;
     }
     static __device__ __forceinline__ uint32_t child(uint32_t e) { return e & kField; }
     // A recorded next sibling: advance() gives it.

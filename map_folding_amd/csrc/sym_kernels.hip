@@ -605,10 +605,9 @@ __device__ __forceinline__ bool dfs_step(uint8_t *lds, const Lane &ln, uint32_t 
         if (leaf) w.leaves += cnt;
         enter = g != 0 && !leaf;
         pop = !enter;
-        if (enter) {
-            kid = mask_ctz(g);
-            ent = Stack::encode_at(kid, g, cnt);
-        }
+        // (Computed by every expanding lane: with no child they are unused.)
+        kid = mask_ctz_any(g);
+        ent = Stack::encode_at(kid, g, cnt);
     }
     // Backtracking, one level per pass while the lane pops: remove the level's
     // child.  A level with more children (recorded or not) stops it, and so
