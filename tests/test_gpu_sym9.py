@@ -26,6 +26,8 @@ SCHEDULES = {
     "resplit": dict(schedule=1),
     "resplit_tight": dict(schedule=1, budget=4, tail=4),
     "resplit_nograph": dict(schedule=1, graph=2),
+    # one page walk at a time (the instance ring orders run on) instead of pairs
+    "resplit_unpaired": dict(schedule=1, core=4),
 }
 
 
