@@ -119,10 +119,13 @@ __device__ __forceinline__ uint32_t mask_ctz(uint64_t v) { return static_cast<ui
 // The same for any mask: an empty one gives the top bit's index (for values
 // computed by lanes that do not use them).
 __device__ __forceinline__ uint32_t mask_ctz_any(uint64_t v) { return mask_ctz(v | (1ull << 63)); }
+// u128: both halves' ctz (each defined for an empty half) and a select, no
+// branch.
 __device__ __forceinline__ uint32_t mask_ctz(unsigned __int128 v) {
-    const uint64_t lo = static_cast<uint64_t>(v);
-    return lo ? static_cast<uint32_t>(__builtin_ctzll(lo))
-              : 64u + static_cast<uint32_t>(__builtin_ctzll(static_cast<uint64_t>(v >> 64)));
+    const uint64_t lo = static_cast<uint64_t>(v), hi = static_cast<uint64_t>(v >> 64);
+    const uint32_t a = static_cast<uint32_t>(__builtin_ctzll(lo | (1ull << 63)));
+    const uint32_t b = 64u + static_cast<uint32_t>(__builtin_ctzll(hi | (1ull << 63)));
+    return lo != 0 ? a : b;
 }
 __device__ __forceinline__ uint32_t mask_ctz_any(unsigned __int128 v) {
     return mask_ctz(v | (static_cast<unsigned __int128>(1) << 127));
