@@ -381,7 +381,7 @@ def lunnon_workload(args, mf, rl, name, steps, warmup):
         if not args.no_tune:
             from map_folding_amd import autotune
             t1 = time.time()
-            cfg = autotune.tune_resplit(front, budgets=(32, 64), tail_mult=(1,), waves=(16, 32), log=log)
+            cfg = autotune.tune_resplit(front, budgets=(32, 64), tail_mult=(1,), waves=(8, 16, 32), log=log)
             batched = mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["tail"],
                                       waves_per_cu=cfg["waves_per_cu"], core=cfg["core"])
             tuned = {k: cfg[k] for k in ("budget", "tail", "waves_per_cu", "core", "kernel_ms")}
