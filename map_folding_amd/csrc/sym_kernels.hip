@@ -555,17 +555,18 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
 // Backtracking pops at most this many levels per step; a lane with more to
 // pop skips the next step's walks and continues.  Bounding the pops keeps one
 // lane's long unwind from holding the whole wave (a lane pops 1.0 levels per
-// node on average, the deepest lane of a wave ~9).  Each popped level costs
-// one or two dependent LDS round trips.  With paired walks a bound of 3 runs
-// the 8x8 depth-36 chunk 1% faster than 4 (5: 3% slower) and 7x7 1.2% faster;
-// 9x9, on one walk per round, is 1.5% faster with 4.  MF_SYM_MAX_POPS > 0
-// overrides the per-geometry bound.
+// node on average, the deepest lane of a wave ~9).  Each popped level is one
+// guarded region with two LDS reads and a write.  With the current pop passes
+// a bound of 4 runs the 8x8 depth-36 chunk 1.3% faster than 3 (2: 5.4%
+// slower, 5 and 6: 1.2% and 2.6% slower than 4), and 9x9 is fastest with 4
+// too; the 7x7 chunk (49-cell rows) is 0.4% faster with 3.  MF_SYM_MAX_POPS
+// > 0 overrides the per-geometry bound.
 #ifndef MF_SYM_MAX_POPS
 #define MF_SYM_MAX_POPS 0
 #endif
 template <typename G>
 constexpr uint32_t max_pops() {
-    return MF_SYM_MAX_POPS > 0 ? MF_SYM_MAX_POPS : (sizeof(typename G::Mask) > 8 ? 4u : 3u);
+    return MF_SYM_MAX_POPS > 0 ? MF_SYM_MAX_POPS : (G::kRowCells <= 49 ? 3u : 4u);
 }
 
 template <typename G, bool EV>
