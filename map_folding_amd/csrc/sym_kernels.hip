@@ -272,7 +272,12 @@ struct Totals {
 #endif
 // The paired loop runs two steps per trip: a closed chain's extra steps
 // change nothing, so the exit test runs every second step (3% less time on
-// the 8x8 and 7x7 chunks than every step; three per trip is slower again).
+// the 8x8 and 7x7 chunks than every step; three and four per trip are 1.9%
+// and 2.6% slower on the 8x8 chunk).
+#ifndef MF_SYM_WALK_UNROLL
+#define MF_SYM_WALK_UNROLL 2
+#endif
+constexpr int kWalkUnroll = MF_SYM_WALK_UNROLL;
 template <typename G>
 constexpr int walk_mode() {
     // (Two lanes per subtree split a level's pages between the pair.)
@@ -485,11 +490,11 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             // as soon as it lands keeps more loads in flight.
             __builtin_amdgcn_s_setprio(1);
             do {
-                // Two paired steps; each chain's two new cells join its
-                // reach in one three-input OR per dword.
-                [[maybe_unused]] Mask bits1[2], bits2[2];
+                // kWalkUnroll paired steps; each chain's new cells join its
+                // reach two at a time in one three-input OR per dword.
+                [[maybe_unused]] Mask bits1[kWalkUnroll], bits2[kWalkUnroll];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (int u = 0; u < kWalkUnroll; ++u) {
                     const uint32_t h1 = hop<G>(r1, i, j1);
                     const uint32_t h2 = hop<G>(r2, i, j2);
                     j1 = cyc_ld<G>(lds, cl, h1);
@@ -510,8 +515,15 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
                     o2 = o2 && j2 != a2;
                 }
                 if constexpr (sizeof(Mask) == 8) {
-                    reach1 = or3(reach1, bits1[0], bits1[1]);
-                    reach2 = or3(reach2, bits2[0], bits2[1]);
+#pragma unroll
+                    for (int u = 0; u + 1 < kWalkUnroll; u += 2) {
+                        reach1 = or3(reach1, bits1[u], bits1[u + 1]);
+                        reach2 = or3(reach2, bits2[u], bits2[u + 1]);
+                    }
+                    if constexpr ((kWalkUnroll & 1) != 0) {
+                        reach1 |= bits1[kWalkUnroll - 1];
+                        reach2 |= bits2[kWalkUnroll - 1];
+                    }
                 }
             } while (o1 || o2);
             __builtin_amdgcn_s_setprio(0);
