@@ -274,6 +274,12 @@ struct Totals {
 // change nothing, so the exit test runs every second step (3% less time on
 // the 8x8 and 7x7 chunks than every step; three and four per trip are 1.9%
 // and 2.6% slower on the 8x8 chunk).
+// Wave priority inside the walk loops (1: 1.4% less time on the 8x8 chunk
+// than 0; 2 and 3 the same as 1).
+#ifndef MF_SYM_WALK_PRIO
+#define MF_SYM_WALK_PRIO 1
+#endif
+constexpr int kWalkPrio = MF_SYM_WALK_PRIO;
 #ifndef MF_SYM_WALK_UNROLL
 #define MF_SYM_WALK_UNROLL 2
 #endif
@@ -329,7 +335,7 @@ __device__ __forceinline__ typename G::Mask page_walk(const uint8_t *__restrict_
     Mask reach = 0;
     or_bit(reach, anchor);
     uint32_t j = cyc_ld<G>(lds, cl, anchor);
-    __builtin_amdgcn_s_setprio(1);   // as in the paired walk (good_mask)
+    __builtin_amdgcn_s_setprio(kWalkPrio);   // as in the paired walk (good_mask)
     while (j != anchor) {
         ++steps;
         j = hop<G>(prow, i, j);
@@ -488,7 +494,7 @@ __device__ __forceinline__ typename G::Mask good_mask(const uint8_t *__restrict_
             // Waves inside a walk loop issue first: their next instruction is
             // usually waiting on an LDS load, so handing them the issue slot
             // as soon as it lands keeps more loads in flight.
-            __builtin_amdgcn_s_setprio(1);
+            __builtin_amdgcn_s_setprio(kWalkPrio);
             do {
                 // kWalkUnroll paired steps; each chain's new cells join its
                 // reach two at a time in one three-input OR per dword.
