@@ -247,7 +247,8 @@ def sym_workload(args, mf, rl, cost, name, steps, warmup, model=None, pairs=None
     core = 3 if cfg.get("lanes", 1) == 2 else 0
     log(f"{name}: tuned deepen {cfg['deepen']} theta {cfg['theta']} cap {cfg['cap']} budget {cfg['budget']} "
         f"lanes per subtree {cfg.get('lanes', 1)} ({cfg['tune_s']:.1f} s)")
-    opts = mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=cfg["budget"], waves_per_cu=32,
+    tail = args.tail if (args.no_tune and args.tail > 0) else cfg["budget"]
+    opts = mf.SolveOptions(count_events=False, budget=cfg["budget"], tail=tail, waves_per_cu=32,
                            deepen=cfg["deepen"], deepen_levels=levels, core=core)
     for w in range(warmup):
         r, _, _ = front.solve(opts)
@@ -540,6 +541,8 @@ def main():
                     help="sym with --no-tune: learned threshold (log2 predicted nodes); unset = no learned targets")
     ap.add_argument("--cap", type=int, default=4, help="sym with --no-tune: most levels the model adds")
     ap.add_argument("--budget", type=int, default=64, help="with --no-tune: re-split budget")
+    ap.add_argument("--tail", type=int, default=0,
+                    help="sym with --no-tune: steps a lane runs after the input is drained (0: the budget)")
     ap.add_argument("--waves", type=int, default=0,
                     help="Lunnon workloads with --no-tune: resident waves per CU (0: the library default)")
     ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
