@@ -7,8 +7,9 @@
 //     across the wave's lanes (entry c of lane L at byte c*S + L, S = 68 for
 //     the u64 instances so that quad lanes reading different cells fall in
 //     different banks, 64 for 9x9);
-//   * the partner of every cell on each of the 4 pages sits in an LDS table
-//     (byte p of word c);
+//   * the partner of every cell on each of the 4 pages sits in LDS tables
+//     (byte p of word c, and page-major bytes for the walks), with one page
+//     word per level naming the one or two pages its get_good walks;
 //   * node = one get_good call (sym_fold9.cpp:342-344); a node at the last
 //     level adds popcount(good) leaves instead of recursing.
 //
@@ -27,8 +28,9 @@
 // Walk semantics: for each page with an earlier neighbour (the anchor), get_good
 // follows cycle successors from the anchor; a cell whose page partner is
 // already placed is replaced by that partner; every cell reached is a legal
-// insertion point; the page masks are intersected.  Each page's walk loop
-// runs until every lane of the wave has closed its walk.
+// insertion point; the page masks are intersected.  A level's two walks run
+// in one loop (walk_mode below), which runs until every lane of the wave has
+// closed both of its walks.
 //
 // Two schedules share the core:
 //   static      one lane per input item, one launch (the plain static-frontier
@@ -270,16 +272,16 @@ struct Totals {
 #ifndef MF_SYM_WALK_MODE
 #define MF_SYM_WALK_MODE -1
 #endif
-// The paired loop runs two steps per trip: a closed chain's extra steps
-// change nothing, so the exit test runs every second step (3% less time on
-// the 8x8 and 7x7 chunks than every step; three and four per trip are 1.9%
-// and 2.6% slower on the 8x8 chunk).
 // Wave priority inside the walk loops (1: 1.4% less time on the 8x8 chunk
 // than 0; 2 and 3 the same as 1).
 #ifndef MF_SYM_WALK_PRIO
 #define MF_SYM_WALK_PRIO 1
 #endif
 constexpr int kWalkPrio = MF_SYM_WALK_PRIO;
+// The paired loop runs two steps per trip: a closed chain's extra steps
+// change nothing, so the exit test runs every second step (3% less time on
+// the 8x8 and 7x7 chunks than every step; three and four per trip are 1.9%
+// and 2.6% slower on the 8x8 chunk).
 #ifndef MF_SYM_WALK_UNROLL
 #define MF_SYM_WALK_UNROLL 2
 #endif
