@@ -9,7 +9,8 @@
 //   * the gap computation of leaf l (mf.c:88-124) walks each dimension's
 //     chain m -> d[i][l][b[m]] and keeps the positions reached by every
 //     dimension that constrains l -- as u64 masks instead of a gap stack and
-//     a per-leaf tally (positions are < l <= 64);
+//     a per-leaf tally (positions are < l <= 64); the two dimensions of a
+//     rectangle walk in one loop (gap_mask);
 //   * every legal position is one insert (node).  An insert at position 0 of
 //     a leaf l >= 2 breaks the normal-form filter b[0] == 1 (mf.c:83) and is
 //     pruned at once, so it is counted but never entered; an insert of leaf n
