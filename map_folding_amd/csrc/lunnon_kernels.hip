@@ -59,13 +59,14 @@ struct LTotals {
     unsigned long long nodes = 0, walks = 0, steps = 0, cands = 0, comps = 0, items = 0;
 };
 
-// Byte stride between entries of a lane's link row: 65, so that lanes of a
-// quad reading different leaves land in different banks (6x6 0.6% and the 64
-// rectangles 1.0% less time than 64; one more byte per entry).  The rows are
-// then addressed by absolute LDS address, as the kernels' only LDS is the
-// dynamic block at offset 0.
+// Byte stride between entries of a lane's link row: 73, so that lanes of a
+// quad reading different leaves land in different banks.  Against 64 it runs
+// the 6x6 depth-18 frontier 2.9% and the 64 rectangles 1.8% faster (65: 0.6%
+// and 1.0%; 77 and 81 the same as 73, 85 slower), and the blocks per CU stay
+// what the registers allow.  The rows are then addressed by absolute LDS
+// address, as the kernels' only LDS is the dynamic block at offset 0.
 #ifndef MF_LUNNON_ROW_STRIDE
-#define MF_LUNNON_ROW_STRIDE 65
+#define MF_LUNNON_ROW_STRIDE 73
 #endif
 constexpr uint32_t kLStride = MF_LUNNON_ROW_STRIDE;
 
