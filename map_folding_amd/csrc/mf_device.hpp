@@ -101,7 +101,7 @@ constexpr unsigned long long kErrDeepenFull = 16ull;
 struct LunnonArgs {
     const uint8_t *tables;      // concatenated d tables, [ndim][n+1][n+1] bytes each
     uint32_t tables_bytes;
-    uint32_t row_entries;       // entries of a lane's LDS row: the largest map's leaf count n + 1 (at most 64)
+    uint32_t row_entries;       // entries of a lane's LDS row: the largest map's leaf count n, plus one for the paired two-dimensional walk (at most 64)
     uint32_t num_maps;
     LunnonMap maps[kMaxMaps];
     const WorkItem *items;
